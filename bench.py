@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark: heap ops/s (mallocs + frees per second) of the MI355X device
+allocator at 524,288 concurrent work-items, batched persistent form vs the
+plain per-launch form.
+
+Workload (full occupancy): 1 GiB native heap of 16 KiB superblocks;
+524,288 work-items (2,048 work-groups x 256); each round every work-item
+mallocs 2^e bytes (e uniform in 3..11, splitmix64 of seed 1), holds it for
+0..64 iterations of one u32 write, reads its pattern back and frees it.
+One step = one round = 1,048,576 heap ops.  Inputs are generated on the
+device from the seed; nothing crosses PCIe in the timed region.
+
+Prints ONE JSON line (rank 0, N = 1).
+"""
+import argparse
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "mallocs+frees/s at 512K work-items, batched vs plain; p99 alloc latency"
+NWI = 524288
+BLOCK = 256
+SB_LOG2 = 14
+SBLOCKS = 65536  # 1 GiB
+SEED = 1
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per heap op: one 4-B bitmap word read-modify-write per
+# malloc or free (8 B), plus the workload payload per work-item-round
+# (pattern write + mean 32 hold writes + read-back, 4 B each) split over
+# its two ops: (1 + 32 + 1) * 4 / 2 = 68 B.
+ALGO_BYTES_PER_OP = 8 + 68
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(rounds=4):
+    """Reference algorithm, CPU restatement (oracle/refmodel.c, pthreads with
+    GCC atomics, same CAS protocol), on this host's cores."""
+    from oracle import refmodel as rm
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    r = rm.cpu_run(SB_LOG2, SBLOCKS, threads, NWI, rounds, 1, 0, SEED)
+    return {
+        "value": r["ops"] / r["seconds"],
+        "unit": "heap ops/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "rounds 0-%d of the full-occupancy trace, all %d work-items (%d ops, %.2f s); "
+                  "reference protocol incl. its kma.cl:338 write-after-release (%d read-back mismatches)"
+                  % (rounds - 1, NWI, r["ops"], r["seconds"], r["bad"]),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every heap is device-local and this build runs on "
+              "one MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import kma_amd
+
+    if kma_amd.lib().kma_device_ok() != 0:
+        raise SystemExit("no GPU: " + kma_amd.last_error())
+
+    K, W = args.steps, args.warmup
+    res = {}
+    for form, name in ((kma_amd.FORM_PLAIN, "plain"), (kma_amd.FORM_BATCHED, "batched")):
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS) as h:
+            log("bench: %s form, %d warmup + %d timed rounds" % (name, W, K))
+            st = h.run(kma_amd.WL_LOGUNI, NWI, K, form=form, block=BLOCK, seed=SEED, warmup=W)
+            st["ops_per_s"] = st["ops"] / (st["gpu_ms"] * 1e-3)
+            res[name] = st
+            log("bench: %s %.3e ops/s, %.3f ms/round, nulls %d bad %d"
+                % (name, st["ops_per_s"], st["gpu_ms"] / K, st["nulls"], st["bad"]))
+
+    import ctypes
+
+    peak = {}
+    for pat, name in ((0, "distinct"), (2, "same_word_per_wave")):
+        ops = ctypes.c_double()
+        ms = ctypes.c_double()
+        rc = kma_amd.lib().kma_atomic_peak(ctypes.c_uint64(1 << 30), ctypes.c_uint64(NWI), 64, pat, 2, 5,
+                                           ctypes.byref(ops), ctypes.byref(ms))
+        if rc != 0:
+            raise SystemExit("atomic peak: " + kma_amd.last_error())
+        peak[name] = ops.value
+        log("bench: atomicOr peak (%s) %.3e /s" % (name, ops.value))
+
+    b, p = res["batched"], res["plain"]
+    value = b["ops_per_s"]
+    dom = b  # dominant kernel: the batched launch
+    ops_per_launch = dom["ops"] / max(1, dom["launches"])
+    achieved = ops_per_launch * ALGO_BYTES_PER_OP / (dom["kernel_ms_avg"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "heap ops/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": b["gpu_ms"] / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64 integer",
+        "data": "synthetic (device-generated splitmix64 trace, seed 1)",
+        "config": {
+            "workload": "full occupancy: 1 GiB heap, 16 KiB superblocks, 524,288 work-items "
+                        "(2,048 x 256), sizes 2^U{3..11} B, hold U{0..64} u32 writes, seed 1",
+            "heap_bytes": 1 << 30,
+            "work_items": NWI,
+            "rounds_timed": K,
+            "form": "batched persistent (one launch, all rounds)",
+        },
+        "plain": {"value": p["ops_per_s"], "ms_per_step": p["gpu_ms"] / K, "launches": p["launches"]},
+        "batched_over_plain": value / p["ops_per_s"],
+        "invariants": {k: int(b[k] + p[k]) for k in ("nulls", "bad", "double_frees", "bad_frees")},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "atomic_roofline": {
+            "peak_distinct_atomics_per_s": peak["distinct"],
+            "peak_same_word_per_wave_atomics_per_s": peak["same_word_per_wave"],
+            "achieved_ops_per_s": value,
+            "frac": value / peak["distinct"],
+        },
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

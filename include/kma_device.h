@@ -1,0 +1,527 @@
+/*
+ * kma_device.h - device-side kma_malloc / kma_free on the native gfx950 heap
+ * layout (HIP).  Header-only so that user kernels can call it.
+ *
+ * Same call surface as the reference's device API (clheap.h:37-39): any lane
+ * may call kma_malloc(heap, size) / kma_free(heap, ptr) on its own, with no
+ * uniformity requirement; NULL on failure; kma_free(heap, NULL) is a no-op.
+ *
+ * Heap layout (one device buffer, all offsets from the header):
+ *   kma_dheap header | u64 state[nsb] | u32 pool_next[nsb] |
+ *   u32 bitmap[nsb][bm_words] | u64 heads[nctx][KMA_HEADS_PER_CTX] |
+ *   pool shards (128 B each) | u64 pbitmap[npages/64] | u8 plen[npages] |
+ *   superblock data (nsb << sb_log2, SB-aligned) | page data.
+ * Metadata lives outside the superblocks, so blocks are naturally aligned to
+ * their (power-of-two) size and user data is never written by the allocator.
+ *
+ * Superblock state word (64-bit, one per superblock):
+ *   bits  0..31  count: live blocks + in-flight reservations
+ *   bits 32..36  class + 1 (0: superblock is free / in the pool)
+ *   bit  37      owned: attached to a context head
+ *   bits 40..63  generation, bumped each time the superblock returns to the pool
+ * A wavefront group of n same-class lanes reserves with ONE atomicAdd(n) on
+ * this word; the returned old value says (atomically) whether the superblock
+ * is still the incarnation the group read from the context head (generation),
+ * how many slots were left (count) and who must return it to the pool.
+ * Whoever performs the atomic that leaves {count 0, not owned, class != 0}
+ * returns the superblock to the pool with a CAS that also bumps the
+ * generation.  Over-reservations are undone with atomicSub; a transiently
+ * inflated count only makes a superblock look fuller than it is.
+ *
+ * Context heads (64-bit): 0 empty, 1 refill in progress, otherwise
+ * (gen << 32) | (sb + 2).  Pool shards are Treiber stacks with a 64-bit
+ * {top, tag} head (ABA tag bumped on every pop and push).
+ * Large requests (> largest class) take 1..64 contiguous pages from a page
+ * bitmap (64-bit words, runs aligned to the next power of two).
+ */
+#ifndef KMA_DEVICE_H
+#define KMA_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KMA_NATIVE_MAGIC 0x4e414d4b31414d4bull /* "KMA1KMAN" */
+#define KMA_HEADS_PER_CTX 32u
+#define KMA_POOL_STRIDE 16u /* u64 words per pool shard (128 B) */
+
+typedef struct kma_dheap {
+	uint64_t magic;
+	uint64_t bytes;
+	uint64_t nsb;
+	uint64_t npages;
+	uint32_t sb_log2;
+	uint32_t page_log2;
+	uint32_t nclasses;
+	uint32_t nctx;
+	uint32_t npool;
+	uint32_t bm_words;
+	uint32_t max_small;
+	uint32_t pad0;
+	uint64_t off_state;
+	uint64_t off_next;
+	uint64_t off_bitmap;
+	uint64_t off_heads;
+	uint64_t off_pool;
+	uint64_t off_pbitmap;
+	uint64_t off_plen;
+	uint64_t off_data;
+	uint64_t off_pages;
+	uint64_t off_stats;
+	uint32_t class_size[32];
+	uint32_t class_cap[32];
+	uint64_t class_magic[32]; /* ceil(2^40 / size): exact block index */
+	uint8_t class_for_log2[64];
+} kma_dheap;
+
+/* Per-heap counters (u64 at off_stats). */
+#define KMA_STAT_DOUBLE_FREE 0
+#define KMA_STAT_BAD_FREE 1
+#define KMA_STAT_POOL_EMPTY 2
+#define KMA_STAT_SPIN_LIMIT 3
+#define KMA_NSTATS 8
+
+namespace kma_nat {
+
+static constexpr uint64_t HEAD_EMPTY = 0;
+static constexpr uint64_t HEAD_BUSY = 1;
+static constexpr uint64_t ST_COUNT = 0xffffffffull;
+static constexpr uint64_t ST_CLASS_SHIFT = 32;
+static constexpr uint64_t ST_CLASS = 0x1full << ST_CLASS_SHIFT;
+static constexpr uint64_t ST_OWNED = 1ull << 37;
+static constexpr uint64_t ST_GEN_SHIFT = 40;
+/* Upper bound on retries of any wait; reaching it means the heap is corrupt:
+ * the call counts KMA_STAT_SPIN_LIMIT and fails instead of spinning forever. */
+static constexpr uint32_t SPIN_LIMIT = 1u << 24;
+
+__device__ __forceinline__ uint32_t st_count(uint64_t s) { return (uint32_t)(s & ST_COUNT); }
+__device__ __forceinline__ uint32_t st_cls1(uint64_t s) { return (uint32_t)((s & ST_CLASS) >> ST_CLASS_SHIFT); }
+__device__ __forceinline__ uint32_t st_gen(uint64_t s) { return (uint32_t)(s >> ST_GEN_SHIFT); }
+
+/* ---- agent-scope atomics ---- */
+template <typename T>
+__device__ __forceinline__ T ld_acq(const T *p)
+{
+	return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_rlx(const T *p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_rel(T *p, T v)
+{
+	__hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T cas(T *p, T cmp, T val)
+{
+	__hip_atomic_compare_exchange_strong(p, &cmp, val, __ATOMIC_ACQ_REL,
+					     __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+	return cmp;
+}
+template <typename T>
+__device__ __forceinline__ T fadd(T *p, T v)
+{
+	return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T fsub(T *p, T v)
+{
+	return __hip_atomic_fetch_sub(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T f_or(T *p, T v)
+{
+	return __hip_atomic_fetch_or(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T f_and(T *p, T v)
+{
+	return __hip_atomic_fetch_and(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* ---- wavefront helpers ---- */
+__device__ __forceinline__ uint32_t lane() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane()) - 1ull; }
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l)
+{
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l)
+{
+	return ((uint64_t)rdlane((uint32_t)(v >> 32), l) << 32) | rdlane((uint32_t)v, l);
+}
+
+/* ---- layout accessors ---- */
+__device__ __forceinline__ char *base(const kma_dheap *H) { return (char *)H; }
+__device__ __forceinline__ uint64_t *state(const kma_dheap *H, uint64_t sb) { return (uint64_t *)(base(H) + H->off_state) + sb; }
+__device__ __forceinline__ uint32_t *pnext(const kma_dheap *H, uint64_t sb) { return (uint32_t *)(base(H) + H->off_next) + sb; }
+__device__ __forceinline__ uint32_t *bitmap(const kma_dheap *H, uint64_t sb) { return (uint32_t *)(base(H) + H->off_bitmap) + sb * H->bm_words; }
+__device__ __forceinline__ uint64_t *head(const kma_dheap *H, uint32_t ctx, uint32_t c) { return (uint64_t *)(base(H) + H->off_heads) + (uint64_t)ctx * KMA_HEADS_PER_CTX + c; }
+__device__ __forceinline__ uint64_t *pool(const kma_dheap *H, uint32_t s) { return (uint64_t *)(base(H) + H->off_pool) + (uint64_t)s * KMA_POOL_STRIDE; }
+__device__ __forceinline__ uint64_t *pbits(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_pbitmap); }
+__device__ __forceinline__ uint8_t *plen(const kma_dheap *H) { return (uint8_t *)(base(H) + H->off_plen); }
+__device__ __forceinline__ char *sbdata(const kma_dheap *H, uint64_t sb) { return base(H) + H->off_data + (sb << H->sb_log2); }
+__device__ __forceinline__ uint64_t *stats(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats); }
+
+__device__ __forceinline__ void bump_stat(const kma_dheap *H, int k)
+{
+	fadd(stats(H) + k, (uint64_t)1);
+}
+
+/* Size -> class: the first class holding 2^(ceil_log2(size)-1)+1, then walk
+ * up (at most the number of classes within one octave). */
+__device__ __forceinline__ int class_of(const kma_dheap *H, uint32_t size)
+{
+	uint32_t e = size <= 1 ? 0 : 32 - __clz(size - 1);
+	int c = H->class_for_log2[e];
+	while (H->class_size[c] < size)
+		c++;
+	return c;
+}
+
+/* Allocation context of the calling wavefront: contiguous waves share one,
+ * so the per-class superblock a context holds is hit by few waves. */
+__device__ __forceinline__ uint32_t ctx_of(const kma_dheap *H)
+{
+	uint32_t wpb = (blockDim.x * blockDim.y * blockDim.z + 63u) >> 6;
+	uint32_t tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+	uint32_t bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+	return (bid * wpb + (tid >> 6)) & (H->nctx - 1u);
+}
+
+/* ---- superblock pool: sharded Treiber stacks, 64-bit {tag:32, top:32} ---- */
+__device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb)
+{
+	uint64_t *hp = pool(H, shard);
+	for (;;) {
+		uint64_t h = ld_acq(hp);
+		st_rel(pnext(H, sb), (uint32_t)h);
+		uint64_t nh = ((h >> 32) + 1ull) << 32 | (uint64_t)(sb + 1u);
+		if (cas(hp, h, nh) == h)
+			break;
+	}
+	fadd(hp + 1, (uint64_t)1);
+}
+
+__device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32_t *out)
+{
+	uint64_t *hp = pool(H, shard);
+	for (;;) {
+		uint64_t h = ld_acq(hp);
+		uint32_t top = (uint32_t)h;
+		if (top == 0)
+			return false;
+		uint32_t nx = ld_acq(pnext(H, top - 1u));
+		uint64_t nh = ((h >> 32) + 1ull) << 32 | (uint64_t)nx;
+		if (cas(hp, h, nh) == h) {
+			fsub(hp + 1, (uint64_t)1);
+			*out = top - 1u;
+			return true;
+		}
+	}
+}
+
+__device__ inline bool pool_pop(const kma_dheap *H, uint32_t ctx, uint32_t *out)
+{
+	uint32_t m = H->npool - 1u, s0 = ctx & m;
+	for (uint32_t i = 0; i < H->npool; i++) {
+		uint32_t s = (s0 + i) & m;
+		if ((uint32_t)ld_rlx(pool(H, s)) == 0)
+			continue;
+		if (pool_pop_shard(H, s, out))
+			return true;
+	}
+	return false;
+}
+
+/* Called by whoever produced state `s` == {count 0, not owned, class set}. */
+__device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t ctx)
+{
+	uint32_t g = st_gen(s);
+	for (;;) {
+		uint64_t fresh = (uint64_t)((g + 1u) & 0xffffffu) << ST_GEN_SHIFT;
+		if (cas(state(H, sb), s, fresh) == s) {
+			pool_push(H, ctx & (H->npool - 1u), sb);
+			return;
+		}
+		s = ld_acq(state(H, sb));
+		if (st_count(s) != 0 || (s & ST_OWNED) || st_cls1(s) == 0 || st_gen(s) != g)
+			return;
+	}
+}
+
+__device__ __forceinline__ void undo(const kma_dheap *H, uint32_t sb, uint32_t m, uint32_t ctx)
+{
+	uint64_t s = fsub(state(H, sb), (uint64_t)m) - m;
+	if (st_count(s) == 0 && !(s & ST_OWNED) && st_cls1(s) != 0)
+		try_release(H, sb, s, ctx);
+}
+
+struct resv {
+	uint32_t sb;
+	uint32_t base;
+	uint32_t got;
+	uint32_t ok; /* 0: pool exhausted */
+};
+
+/* Reserve up to n slots of class c from context ctx's current superblock
+ * (one lane executes this for its wavefront group). */
+__device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uint32_t n)
+{
+	uint64_t *hp = head(H, ctx, c);
+	uint32_t cap = H->class_cap[c];
+	for (uint32_t spin = 0;; spin++) {
+		if (spin >= SPIN_LIMIT) {
+			bump_stat(H, KMA_STAT_SPIN_LIMIT);
+			return resv{0, 0, 0, 0};
+		}
+		uint64_t h = ld_acq(hp);
+		if (h == HEAD_EMPTY) {
+			if (cas(hp, HEAD_EMPTY, HEAD_BUSY) != HEAD_EMPTY)
+				continue;
+			uint32_t sb;
+			if (!pool_pop(H, ctx, &sb)) {
+				st_rel(hp, HEAD_EMPTY);
+				bump_stat(H, KMA_STAT_POOL_EMPTY);
+				return resv{0, 0, 0, 0};
+			}
+			uint64_t add = ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ST_OWNED | n;
+			uint64_t old = fadd(state(H, sb), add);
+			uint32_t cnt = st_count(old);
+			uint32_t got = cnt >= cap ? 0u : min(n, cap - cnt);
+			if (got < n)
+				fsub(state(H, sb), (uint64_t)(n - got));
+			st_rel(hp, ((uint64_t)st_gen(old) << 32) | (uint64_t)(sb + 2u));
+			if (got == 0)
+				continue;
+			return resv{sb, cnt, got, 1};
+		}
+		if (h == HEAD_BUSY) {
+			__builtin_amdgcn_s_sleep(2);
+			continue;
+		}
+		uint32_t sb = (uint32_t)h - 2u;
+		uint32_t g = (uint32_t)(h >> 32);
+		uint64_t old = fadd(state(H, sb), (uint64_t)n);
+		if (st_gen(old) != g) {
+			undo(H, sb, n, ctx);
+			continue;
+		}
+		uint32_t cnt = st_count(old);
+		uint32_t got = cnt >= cap ? 0u : min(n, cap - cnt);
+		if (got < n) {
+			undo(H, sb, n - got, ctx);
+			if (cas(hp, h, HEAD_EMPTY) == h) {
+				uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
+				if (st_count(s) == 0 && st_cls1(s) != 0)
+					try_release(H, sb, s, ctx);
+			}
+			if (got == 0)
+				continue;
+		}
+		return resv{sb, cnt, got, 1};
+	}
+}
+
+/* Claim one free bit at or after `start` (wrapping) in superblock sb. */
+__device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t start)
+{
+	uint32_t cap = H->class_cap[c];
+	uint32_t *bm = bitmap(H, sb);
+	uint32_t b = start;
+	uint32_t lastw = (cap - 1u) >> 5;
+	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
+	for (uint32_t spin = 0; spin < SPIN_LIMIT; spin++) {
+		uint32_t wi = b >> 5;
+		uint32_t w = ld_rlx(bm + wi);
+		uint32_t avail = ~w & (~0u << (b & 31u));
+		if (wi == lastw)
+			avail &= tail;
+		if (avail) {
+			uint32_t bit = __ffs(avail) - 1u;
+			uint32_t m = 1u << bit;
+			if (!(f_or(bm + wi, m) & m))
+				return sbdata(H, sb) + (uint64_t)(wi * 32u + bit) * H->class_size[c];
+			b = wi * 32u + bit;
+			continue;
+		}
+		b = (wi + 1u) << 5;
+		if (b >= cap)
+			b = 0;
+	}
+	bump_stat(H, KMA_STAT_SPIN_LIMIT);
+	return nullptr;
+}
+
+/* ---- large requests: page runs ---- */
+__device__ __forceinline__ uint64_t aligned_free_runs(uint64_t w, uint32_t kp)
+{
+	uint64_t z = ~w;
+	for (uint32_t s = 1; s < kp; s <<= 1)
+		z &= z >> s;
+	uint64_t pat = 0;
+	for (uint32_t i = 0; i < 64; i += kp)
+		pat |= 1ull << i;
+	return z & pat;
+}
+
+__device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t seed)
+{
+	uint64_t psz = 1ull << H->page_log2;
+	uint64_t k = (size + psz - 1) >> H->page_log2;
+	if (k > 64 || H->npages == 0)
+		return nullptr;
+	uint32_t kp = 1;
+	while (kp < k)
+		kp <<= 1;
+	uint64_t run = (k == 64) ? ~0ull : ((1ull << k) - 1ull);
+	uint64_t nwords = (H->npages + 63) >> 6;
+	uint64_t *pb = pbits(H);
+	uint64_t wi = ((uint64_t)seed * 0x9E3779B1u) % nwords;
+	for (uint64_t probe = 0; probe < nwords;) {
+		uint64_t w = ld_rlx(pb + wi);
+		uint64_t cand = aligned_free_runs(w, kp);
+		if (wi == nwords - 1 && (H->npages & 63)) {
+			/* runs must end inside the page range */
+			uint32_t valid = (uint32_t)(H->npages & 63);
+			uint64_t ok = 0;
+			for (uint32_t j = 0; j + k <= valid; j += kp)
+				ok |= 1ull << j;
+			cand &= ok;
+		}
+		if (!cand) {
+			wi = (wi + 1 == nwords) ? 0 : wi + 1;
+			probe++;
+			continue;
+		}
+		uint32_t j = __ffsll((unsigned long long)cand) - 1u;
+		uint64_t m = run << j;
+		uint64_t old = f_or(pb + wi, m);
+		if ((old & m) == 0) {
+			uint64_t pi = (wi << 6) + j;
+			__hip_atomic_store(plen(H) + pi, (uint8_t)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			return base(H) + H->off_pages + (pi << H->page_log2);
+		}
+		if (m & ~old)
+			f_and(pb + wi, ~(m & ~old));
+	}
+	return nullptr;
+}
+
+__device__ inline void page_free(const kma_dheap *H, char *p)
+{
+	uint64_t pi = (uint64_t)(p - (base(H) + H->off_pages)) >> H->page_log2;
+	uint32_t k = __hip_atomic_load(plen(H) + pi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	if (k == 0 || k > 64) {
+		bump_stat(H, KMA_STAT_BAD_FREE);
+		return;
+	}
+	uint64_t m = ((k == 64) ? ~0ull : ((1ull << k) - 1ull)) << (pi & 63);
+	uint64_t old = f_and(pbits(H) + (pi >> 6), ~m);
+	if ((old & m) != m)
+		bump_stat(H, KMA_STAT_DOUBLE_FREE);
+}
+
+} /* namespace kma_nat */
+
+/* Allocate `size` bytes; NULL for size 0 or when the heap cannot serve it. */
+__device__ inline void *kma_malloc(kma_dheap *H, size_t size)
+{
+	using namespace kma_nat;
+	if (size == 0)
+		return nullptr;
+	if (size > H->max_small) {
+		uint32_t seed = (blockIdx.x * blockDim.x + threadIdx.x) ^ (uint32_t)__builtin_amdgcn_s_memtime();
+		return page_alloc(H, size, seed);
+	}
+	uint32_t c = (uint32_t)class_of(H, (uint32_t)size);
+	uint32_t ctx = ctx_of(H);
+	char *result = nullptr;
+	bool pending = true;
+	for (;;) {
+		uint64_t act = __ballot(pending);
+		if (act == 0)
+			break;
+		uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+		uint32_t lc = rdlane(c, leader);
+		bool member = pending && c == lc;
+		uint64_t grp = __ballot(member);
+		uint32_t n = (uint32_t)__popcll(grp);
+		resv r = {0, 0, 0, 0};
+		if (lane() == leader)
+			r = reserve(H, ctx, lc, n);
+		uint32_t sb = rdlane(r.sb, leader);
+		uint32_t rbase = rdlane(r.base, leader);
+		uint32_t got = rdlane(r.got, leader);
+		uint32_t ok = rdlane(r.ok, leader);
+		if (member) {
+			uint32_t q = (uint32_t)__popcll(grp & lanemask_lt());
+			if (!ok) {
+				pending = false;
+			} else if (q < got) {
+				result = claim(H, sb, c, rbase + q);
+				pending = false;
+			}
+		}
+	}
+	return result;
+}
+
+/* Free a block obtained from kma_malloc on the same heap; NULL is a no-op.
+ * A second free of the same block is detected and counted, not applied. */
+__device__ inline void kma_free(kma_dheap *H, void *ptr)
+{
+	using namespace kma_nat;
+	if (!ptr)
+		return;
+	char *p = (char *)ptr;
+	char *d0 = base(H) + H->off_data;
+	uint64_t off = (uint64_t)(p - d0);
+	if (p < d0 || off >= (H->nsb << H->sb_log2)) {
+		if (H->npages && p >= base(H) + H->off_pages)
+			page_free(H, p);
+		else
+			bump_stat(H, KMA_STAT_BAD_FREE);
+		return;
+	}
+	uint32_t sb = (uint32_t)(off >> H->sb_log2);
+	uint64_t s = ld_rlx(state(H, sb));
+	uint32_t c1 = st_cls1(s);
+	bool live = c1 != 0;
+	if (live) {
+		uint64_t inoff = off & ((1ull << H->sb_log2) - 1ull);
+		uint32_t idx = (uint32_t)((inoff * H->class_magic[c1 - 1u]) >> 40);
+		uint32_t m = 1u << (idx & 31u);
+		uint32_t old = f_and(bitmap(H, sb) + (idx >> 5), ~m);
+		if (!(old & m)) {
+			bump_stat(H, KMA_STAT_DOUBLE_FREE);
+			live = false;
+		}
+	} else {
+		bump_stat(H, KMA_STAT_BAD_FREE);
+	}
+	/* one count update per superblock per wavefront */
+	uint32_t ctx = ctx_of(H);
+	bool pending = live;
+	for (;;) {
+		uint64_t act = __ballot(pending);
+		if (act == 0)
+			break;
+		uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+		uint32_t lsb = rdlane(sb, leader);
+		bool member = pending && sb == lsb;
+		uint64_t grp = __ballot(member);
+		if (lane() == leader) {
+			uint32_t n = (uint32_t)__popcll(grp);
+			uint64_t ns = fsub(state(H, lsb), (uint64_t)n) - n;
+			if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
+				try_release(H, lsb, ns, ctx);
+		}
+		if (member)
+			pending = false;
+	}
+}
+
+#endif /* KMA_DEVICE_H */

@@ -1,0 +1,213 @@
+"""kma_amd - Python view of libkma.so, the MI355X device allocator.
+
+Thin ctypes binding over the C ABI declared in include/kma.h and
+include/kma_work.h.  No torch: device memory and streams are owned by the
+library.  Loading fails loudly if libkma.so has not been built; there is no
+CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkma.so")
+
+LAYOUT_REFERENCE = 0
+LAYOUT_NATIVE = 1
+MAX_CLASSES = 24
+
+WL_FIXED, WL_LOGUNI, WL_REF_TEST, WL_LOWVAR, WL_HIGHVAR = range(5)
+FORM_PLAIN, FORM_BATCHED = 0, 1
+
+# Exported symbols, as declared in include/*.h (checked by the CPU tests).
+EXPORTS = [
+    "kma_create", "kma_create_ex", "kma_destroy", "kma_device_heap",
+    "kma_heap_bytes", "kma_get_config", "kma_snapshot", "kma_reset",
+    "kma_last_error", "kma_device_ok", "kma_run_workload", "kma_trace_serial",
+    "kma_trim",
+]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("layout", ctypes.c_uint32),
+        ("sb_log2", ctypes.c_uint32),
+        ("sblocks", ctypes.c_uint64),
+        ("nctx", ctypes.c_uint32),
+        ("npool", ctypes.c_uint32),
+        ("nclasses", ctypes.c_uint32),
+        ("class_size", ctypes.c_uint32 * MAX_CLASSES),
+        ("page_log2", ctypes.c_uint32),
+        ("npages", ctypes.c_uint64),
+    ]
+
+
+class Workload(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_uint32),
+        ("form", ctypes.c_uint32),
+        ("nwi", ctypes.c_uint64),
+        ("block", ctypes.c_uint32),
+        ("rounds", ctypes.c_uint32),
+        ("round0", ctypes.c_uint32),
+        ("fixed", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("warmup", ctypes.c_uint32),
+        ("reps", ctypes.c_uint32),
+    ]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("gpu_ms", ctypes.c_double),
+        ("kernel_ms_avg", ctypes.c_double),
+        ("launches", ctypes.c_uint64),
+        ("ops", ctypes.c_uint64),
+        ("mallocs", ctypes.c_uint64),
+        ("nulls", ctypes.c_uint64),
+        ("bad", ctypes.c_uint64),
+        ("double_frees", ctypes.c_uint64),
+        ("bad_frees", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KmaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libkma.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KmaError("libkma.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+        P = ctypes.POINTER
+        L.kma_create.argtypes = [ctypes.c_uint]
+        L.kma_create.restype = vp
+        L.kma_create_ex.argtypes = [P(Config)]
+        L.kma_create_ex.restype = vp
+        L.kma_destroy.argtypes = [vp]
+        L.kma_destroy.restype = None
+        L.kma_device_heap.argtypes = [vp]
+        L.kma_device_heap.restype = vp
+        L.kma_heap_bytes.argtypes = [vp]
+        L.kma_heap_bytes.restype = u64
+        L.kma_get_config.argtypes = [vp, P(Config)]
+        L.kma_snapshot.argtypes = [vp, u64, vp, u64]
+        L.kma_reset.argtypes = [vp]
+        L.kma_last_error.restype = ctypes.c_char_p
+        L.kma_device_ok.restype = ctypes.c_int
+        L.kma_run_workload.argtypes = [vp, P(Workload), P(RunStats)]
+        L.kma_trace_serial.argtypes = [vp, vp, u64, vp, vp]
+        L.kma_trim.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().kma_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise KmaError("%s failed: %s" % (what, last_error()))
+
+
+class Heap:
+    """A device heap (kma_create / kma_create_ex)."""
+
+    def __init__(self, layout=LAYOUT_REFERENCE, sb_log2=12, sblocks=4096, nctx=0,
+                 npool=0, classes=None, page_log2=0, npages=0):
+        c = Config()
+        c.layout = layout
+        c.sb_log2 = sb_log2
+        c.sblocks = sblocks
+        c.nctx = nctx
+        c.npool = npool
+        if classes:
+            c.nclasses = len(classes)
+            for i, s in enumerate(classes):
+                c.class_size[i] = s
+        c.page_log2 = page_log2
+        c.npages = npages
+        self._h = lib().kma_create_ex(ctypes.byref(c))
+        if not self._h:
+            raise KmaError("kma_create_ex: " + last_error())
+        self.layout = layout
+
+    @classmethod
+    def create(cls, sblocks):
+        """Reference-compatible constructor: kma_create(sblocks)."""
+        self = cls.__new__(cls)
+        self._h = lib().kma_create(sblocks)
+        if not self._h:
+            raise KmaError("kma_create: " + last_error())
+        self.layout = LAYOUT_REFERENCE
+        return self
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_ptr(self):
+        return lib().kma_device_heap(self._h)
+
+    @property
+    def nbytes(self):
+        return lib().kma_heap_bytes(self._h)
+
+    def config(self):
+        c = Config()
+        _check(lib().kma_get_config(self._h, ctypes.byref(c)), "kma_get_config")
+        return c
+
+    def snapshot(self, offset=0, nbytes=None):
+        if nbytes is None:
+            nbytes = self.nbytes - offset
+        buf = np.empty(nbytes, dtype=np.uint8)
+        _check(lib().kma_snapshot(self._h, offset, buf.ctypes.data, nbytes), "kma_snapshot")
+        return buf
+
+    def reset(self):
+        _check(lib().kma_reset(self._h), "kma_reset")
+
+    def trim(self):
+        _check(lib().kma_trim(self._h), "kma_trim")
+
+    def trace_serial(self, ops, pat=None):
+        ops = np.ascontiguousarray(ops, dtype=np.int64)
+        out = np.zeros(len(ops), dtype=np.uint64)
+        patp = None
+        if pat is not None:
+            pat = np.ascontiguousarray(pat, dtype=np.uint32)
+            patp = pat.ctypes.data
+        _check(lib().kma_trace_serial(self._h, ops.ctypes.data, len(ops), out.ctypes.data, patp),
+               "kma_trace_serial")
+        return out
+
+    def run(self, mode, nwi, rounds, form=FORM_PLAIN, block=64, fixed=64, seed=1,
+            round0=0, warmup=0, reps=1):
+        w = Workload(mode, form, nwi, block, rounds, round0, fixed, seed, warmup, reps)
+        st = RunStats()
+        _check(lib().kma_run_workload(self._h, ctypes.byref(w), ctypes.byref(st)), "kma_run_workload")
+        return st.as_dict()

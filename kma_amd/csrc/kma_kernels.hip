@@ -1,0 +1,355 @@
+/*
+ * kma_kernels.hip - gfx950 kernels behind libkma.so: heap initialisation,
+ * serialized traces and the per-launch ("plain") workloads.
+ */
+#include <hip/hip_runtime.h>
+
+#include "kma_device.h"
+#include "kma_launch.h"
+#include "kma_ref_device.h"
+#include "kma_work.h"
+
+#define KMA_CHECK(x)                                   \
+	do {                                           \
+		hipError_t e_ = (x);                   \
+		if (e_ != hipSuccess)                  \
+			return -(int)e_;               \
+	} while (0)
+
+/* ------------------------------------------------------------------ */
+/* deterministic trace values (same function as the host side)         */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ uint64_t kma_mix(uint64_t seed, uint64_t gid, uint64_t round, uint64_t k)
+{
+	uint64_t z = seed * 0x9E3779B97F4A7C15ull + gid * 0xBF58476D1CE4E5B9ull +
+		     round * 0x94D049BB133111EBull + k * 0xD6E8FEB86659FD93ull;
+	z += 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+/* ------------------------------------------------------------------ */
+/* reference layout: parallel heap init                                 */
+/* ------------------------------------------------------------------ */
+/*
+ * The reference's clheap_init (kma.cl:27-52) enqueues superblocks 1..S-2
+ * one by one behind the dummy sb[0] from a single work-item.  On a
+ * zero-filled buffer every word that chain of enqueues writes has a closed
+ * form, so each thread writes one superblock's link word directly:
+ *   sb[0].next   = PTR(idx(sb1), 1)            (tag 1 from the queue init)
+ *   sb[i].next   = PTR(idx(sb[i+1]), 0)        for 1 <= i <= S-3
+ *   sb[S-2].next = 0
+ *   head = PTR(idx(sb0), 0),  tail = PTR(idx(sb[S-2]), S-2)
+ * where PTR(i, t) = (i & 0x1fffe) | ((t+1) & 0x7ff) << 22 and
+ * idx(sb[k]) = (k+1) << 1 (clIndexedQueue.cl:44-65,89).
+ */
+__global__ void k_ref_init(char *heap, uint32_t log2, uint32_t S)
+{
+	using namespace kma_ref;
+	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t buckets = log2 - 1u;
+	uint32_t hdr = 32u + 8u * buckets;
+	char *sb0 = heap + hdr;
+	if (i == 0) {
+		uint64_t bytes = ((uint64_t)S << log2) + hdr;
+		*reinterpret_cast<uint64_t *>(heap + 0) = bytes;
+		*reinterpret_cast<uint64_t *>(heap + 8) = (uint64_t)(uintptr_t)sb0;
+		*reinterpret_cast<uint64_t *>(heap + 16) = log2;
+		*reinterpret_cast<uint32_t *>(heap + 24) = q_ptr(2u, 0u);
+		*reinterpret_cast<uint32_t *>(heap + 28) = q_ptr(2u * (S - 1u), S - 2u);
+		for (uint32_t c = 0; c < buckets; c++)
+			*reinterpret_cast<uint64_t *>(heap + 32 + 8 * c) = 0;
+	}
+	if (i + 1 < S) { /* nodes sb[0..S-2] */
+		uint32_t w;
+		if (i + 2 == S)
+			w = (i == 0) ? q_ptr(0u, 0u) : 0u;
+		else if (i == 0)
+			w = q_ptr(4u, 1u);
+		else
+			w = q_ptr((uint32_t)(2u * (i + 2u)), 0u);
+		*reinterpret_cast<uint32_t *>(sb0 + (i << log2)) = w;
+	}
+}
+
+extern "C" int kmad_ref_init(void *heap, uint32_t log2, uint32_t sblocks)
+{
+	uint32_t threads = 256;
+	uint32_t blocks = (sblocks + threads - 1) / threads;
+	hipLaunchKernelGGL(k_ref_init, dim3(blocks), dim3(threads), 0, 0, (char *)heap, log2, sblocks);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* native layout: metadata init                                         */
+/* ------------------------------------------------------------------ */
+/* Pool shard s starts as the stack s, s+npool, s+2*npool, ... (lowest on
+ * top); every other metadata word starts at zero (done by the host memset). */
+__global__ void k_nat_init(kma_dheap *H, uint64_t nsb, uint32_t npool)
+{
+	using namespace kma_nat;
+	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < nsb) {
+		uint64_t nx = i + npool;
+		*pnext(H, i) = nx < nsb ? (uint32_t)(nx + 1u) : 0u;
+	}
+	if (i < npool) {
+		uint64_t *hp = pool(H, (uint32_t)i);
+		hp[0] = i < nsb ? (uint64_t)(i + 1u) : 0u;
+		hp[1] = i < nsb ? (nsb - i + npool - 1u) / npool : 0u;
+	}
+}
+
+extern "C" int kmad_nat_init(void *heap, uint64_t nsb, uint32_t npool, uint64_t nheads, uint64_t npbwords)
+{
+	(void)nheads;
+	(void)npbwords;
+	uint64_t n = nsb > npool ? nsb : npool;
+	uint32_t threads = 256;
+	uint64_t blocks = (n + threads - 1) / threads;
+	hipLaunchKernelGGL(k_nat_init, dim3((uint32_t)blocks), dim3(threads), 0, 0, (kma_dheap *)heap, nsb, npool);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}
+
+/* Detach every context head (no allocation may be in flight). */
+__global__ void k_trim(kma_dheap *H, uint32_t nctx, uint32_t nclasses)
+{
+	using namespace kma_nat;
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nctx * nclasses)
+		return;
+	uint32_t ctx = i / nclasses, c = i % nclasses;
+	uint64_t h = __hip_atomic_exchange(head(H, ctx, c), HEAD_EMPTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+	if (h <= HEAD_BUSY)
+		return;
+	uint32_t sb = (uint32_t)h - 2u;
+	uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
+	if (st_count(s) == 0 && st_cls1(s) != 0)
+		try_release(H, sb, s, ctx);
+}
+
+extern "C" int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses)
+{
+	uint32_t n = nctx * nclasses;
+	hipLaunchKernelGGL(k_trim, dim3((n + 255) / 256), dim3(256), 0, 0, (kma_dheap *)heap, nctx, nclasses);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* serialized trace: one work-item replays the op list                  */
+/* ------------------------------------------------------------------ */
+template <class HeapT>
+__global__ void k_serial(HeapT *heap, const int64_t *ops, uint64_t n, uint64_t *out, const uint32_t *pat)
+{
+	if (blockIdx.x != 0 || threadIdx.x != 0)
+		return;
+	char *hb = reinterpret_cast<char *>(heap);
+	for (uint64_t i = 0; i < n; i++) {
+		int64_t op = ops[i];
+		if (op >= 0) {
+			char *p = (char *)kma_malloc(heap, (size_t)op);
+			out[i] = p ? (uint64_t)(p - hb) : 0;
+			if (p && pat)
+				*reinterpret_cast<volatile uint32_t *>(p) = pat[i];
+		} else {
+			uint64_t off = out[(uint64_t)(-op - 1)];
+			kma_free(heap, off ? (void *)(hb + off) : nullptr);
+			out[i] = 0;
+		}
+	}
+}
+
+extern "C" int kmad_serial(int layout, void *heap, const int64_t *d_ops, uint64_t n, uint64_t *d_out,
+			   const uint32_t *d_pat)
+{
+	if (layout == 0)
+		hipLaunchKernelGGL(k_serial<clheap>, dim3(1), dim3(64), 0, 0, (clheap *)heap, d_ops, n, d_out, d_pat);
+	else
+		hipLaunchKernelGGL(k_serial<kma_dheap>, dim3(1), dim3(64), 0, 0, (kma_dheap *)heap, d_ops, n, d_out, d_pat);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* plain per-launch workloads                                           */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ void wave_add_stats(unsigned long long *stats, uint64_t a, uint64_t b, uint64_t c,
+					       uint64_t d)
+{
+	/* one atomic per counter per wavefront */
+	uint64_t v[4] = {a, b, c, d};
+	for (int k = 0; k < 4; k++) {
+		uint64_t x = v[k];
+		for (int off = 32; off > 0; off >>= 1)
+			x += __shfl_xor(x, off);
+		if (__lane_id() == 0 && x)
+			atomicAdd(stats + k, (unsigned long long)x);
+	}
+}
+
+/*
+ * One kernel body for every per-work-item workload:
+ *  KMA_WL_FIXED    : malloc(fixed), write {gid, round}, read back, free
+ *                    (plain small workload)
+ *  KMA_WL_LOGUNI   : malloc(2^e), e uniform in 3..11, hold 0..64 writes,
+ *                    read back, free (full-occupancy workload)
+ *  KMA_WL_REF_TEST : kma_test_malloc (kma.cl:424-461): malloc(8), write pid,
+ *                    free; a NULL ends the work-item
+ *  KMA_WL_LOWVAR   : kma_test_malloc_lowvar (kma.cl:463-492)
+ *  KMA_WL_HIGHVAR  : kma_test_malloc_highvar (kma.cl:494-526)
+ * stats: [0] heap ops issued, [1] NULL returns, [2] read-back mismatches,
+ *        [3] mallocs issued.
+ */
+template <class HeapT>
+__global__ void __launch_bounds__(256) k_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds, int mode,
+						uint32_t fixed, uint64_t seed, uint64_t nwi,
+						unsigned long long *stats)
+{
+	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
+	bool alive = gid < nwi;
+	for (uint32_t r = round0; r < round0 + nrounds; r++) {
+		if (!alive)
+			break;
+		if (mode == KMA_WL_FIXED || mode == KMA_WL_LOGUNI) {
+			uint32_t size, hold = 0;
+			if (mode == KMA_WL_FIXED) {
+				size = fixed;
+			} else {
+				size = 1u << (3 + kma_mix(seed, gid, r, 0) % 9);
+				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
+			}
+			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, size);
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+				continue;
+			}
+			uint32_t pat = (uint32_t)(gid << 8) ^ r;
+			if (mode == KMA_WL_FIXED) {
+				p[0] = (uint32_t)gid;
+				p[1] = r;
+				if (p[0] != (uint32_t)gid || p[1] != r)
+					bad++;
+			} else {
+				uint32_t words = size >> 2;
+				p[0] = pat;
+				for (uint32_t k = 0; k < hold; k++)
+					p[1 + k % (words - 1)] = pat + k + 1;
+				if (p[0] != pat)
+					bad++;
+			}
+			kma_free(heap, (void *)p);
+			ops++;
+		} else if (mode == KMA_WL_REF_TEST) {
+			volatile uint64_t *p = (volatile uint64_t *)kma_malloc(heap, 8);
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+				alive = false;
+				continue;
+			}
+			p[0] = gid;
+			kma_free(heap, (void *)p);
+			ops++;
+		} else {
+			uint32_t amount;
+			if (mode == KMA_WL_LOWVAR)
+				amount = (gid & 1) ? 8 : 16;
+			else
+				amount = 4u << ((gid + r) % 5);
+			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, amount);
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+				alive = false;
+				continue;
+			}
+			p[0] = mode == KMA_WL_LOWVAR ? (uint32_t)gid : 4919u;
+			if (gid == 0) {
+				volatile uint32_t *q = (volatile uint32_t *)kma_malloc(heap, 4000);
+				mallocs++;
+				ops++;
+				if (!q) {
+					nulls++;
+					alive = false;
+					continue;
+				}
+				q[8] = 4919;
+				kma_free(heap, (void *)q);
+				ops++;
+			}
+			kma_free(heap, (void *)p);
+			ops++;
+		}
+	}
+	wave_add_stats(stats, ops, nulls, bad, mallocs);
+}
+
+extern "C" int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed,
+			   uint64_t seed, uint64_t nwi, uint32_t block, unsigned long long *d_stats, void *stream)
+{
+	if (block == 0 || block > 256 || (block & 63))
+		return -1;
+	uint64_t blocks = (nwi + block - 1) / block;
+	hipStream_t s = (hipStream_t)stream;
+	if (layout == 0)
+		hipLaunchKernelGGL(k_rounds<clheap>, dim3((uint32_t)blocks), dim3(block), 0, s, (clheap *)heap,
+				   round0, nrounds, mode, fixed, seed, nwi, d_stats);
+	else
+		hipLaunchKernelGGL(k_rounds<kma_dheap>, dim3((uint32_t)blocks), dim3(block), 0, s,
+				   (kma_dheap *)heap, round0, nrounds, mode, fixed, seed, nwi, d_stats);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* roofline microbenchmarks                                             */
+/* ------------------------------------------------------------------ */
+/*
+ * Agent-scope atomicOr throughput, the operation the allocator's bitmaps
+ * use.  pattern 0: every lane hits a pseudo-random distinct 4-B word over
+ * the whole buffer (distinct-address peak); 1: lanes of a wavefront hit
+ * consecutive words (coalesced); 2: all lanes of a wavefront hit one word
+ * (contended).  Each lane issues `iters` atomics.
+ */
+__global__ void __launch_bounds__(256) k_atomic_peak(uint32_t *buf, uint64_t nwords, uint32_t iters, int pattern)
+{
+	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
+	uint32_t acc = 0;
+	for (uint32_t i = 0; i < iters; i++) {
+		uint64_t w;
+		if (pattern == 0)
+			w = kma_mix(7, gid, i, 0) % nwords;
+		else if (pattern == 1)
+			w = (gid + (uint64_t)i * nth) % nwords;
+		else
+			w = ((gid >> 6) * 64 + (uint64_t)i * 4099) % nwords;
+		acc ^= __hip_atomic_fetch_or(buf + w, 1u << (i & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+	if (acc == 0x12345678u)
+		buf[0] = acc; /* keep the results live */
+}
+
+extern "C" int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, uint32_t iters, int pattern,
+				void *stream)
+{
+	uint32_t block = 256;
+	uint64_t blocks = (nthreads + block - 1) / block;
+	hipLaunchKernelGGL(k_atomic_peak, dim3((uint32_t)blocks), dim3(block), 0, (hipStream_t)stream,
+			   (uint32_t *)buf, nwords, iters, pattern);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}

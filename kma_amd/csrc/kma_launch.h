@@ -1,0 +1,39 @@
+/*
+ * kma_launch.h - internal C prototypes between the host C layer (kma_host.c)
+ * and the HIP kernels (kma_kernels.hip).  Plain C types only.
+ * Every function returns 0 or a negative hipError_t value.
+ */
+#ifndef KMA_LAUNCH_H
+#define KMA_LAUNCH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reference layout: parallel clheap_init producing the serial init's image. */
+int kmad_ref_init(void *heap, uint32_t log2, uint32_t sblocks);
+
+/* Native layout: metadata init (header already copied to device). */
+int kmad_nat_init(void *heap, uint64_t nsb, uint32_t npool, uint64_t nheads,
+		  uint64_t npbwords);
+
+/* Serialized trace on one work-item. layout selects the device API. */
+int kmad_serial(int layout, void *heap, const int64_t *d_ops, uint64_t n,
+		uint64_t *d_out, const uint32_t *d_pat);
+
+/* Workload kernels: see kma_work.h for the parameters. */
+int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds,
+		int mode, uint32_t fixed, uint64_t seed, uint64_t nwi,
+		uint32_t block, unsigned long long *d_stats, void *stream);
+
+int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses);
+
+int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, uint32_t iters,
+		     int pattern, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,134 @@
+"""Decoder for native-layout heap images (kma_dheap in include/kma_device.h).
+Used by the tests only."""
+import numpy as np
+
+HDR_DTYPE = np.dtype([
+    ("magic", "<u8"), ("bytes", "<u8"), ("nsb", "<u8"), ("npages", "<u8"),
+    ("sb_log2", "<u4"), ("page_log2", "<u4"), ("nclasses", "<u4"), ("nctx", "<u4"),
+    ("npool", "<u4"), ("bm_words", "<u4"), ("max_small", "<u4"), ("pad0", "<u4"),
+    ("off_state", "<u8"), ("off_next", "<u8"), ("off_bitmap", "<u8"), ("off_heads", "<u8"),
+    ("off_pool", "<u8"), ("off_pbitmap", "<u8"), ("off_plen", "<u8"), ("off_data", "<u8"),
+    ("off_pages", "<u8"), ("off_stats", "<u8"),
+    ("class_size", "<u4", 32), ("class_cap", "<u4", 32), ("class_magic", "<u8", 32),
+    ("class_for_log2", "u1", 64),
+])
+
+MAGIC = 0x4E414D4B31414D4B
+HEADS_PER_CTX = 32
+POOL_STRIDE = 16  # u64 words
+
+
+class NatHeap:
+    def __init__(self, heap):
+        raw = heap.snapshot(0, HDR_DTYPE.itemsize)
+        self.h = np.frombuffer(raw.tobytes(), dtype=HDR_DTYPE)[0]
+        assert int(self.h["magic"]) == MAGIC
+        self.heap = heap
+        self.nsb = int(self.h["nsb"])
+        self.npages = int(self.h["npages"])
+        self.sb_log2 = int(self.h["sb_log2"])
+        self.page_log2 = int(self.h["page_log2"])
+        self.nclasses = int(self.h["nclasses"])
+        self.nctx = int(self.h["nctx"])
+        self.npool = int(self.h["npool"])
+        self.bm_words = int(self.h["bm_words"])
+        self.max_small = int(self.h["max_small"])
+        self.classes = [int(x) for x in self.h["class_size"][: self.nclasses]]
+        self.caps = [int(x) for x in self.h["class_cap"][: self.nclasses]]
+        self.off_data = int(self.h["off_data"])
+        self.off_pages = int(self.h["off_pages"])
+
+    def meta(self):
+        """Metadata snapshot (everything before the superblock data)."""
+        img = self.heap.snapshot(0, int(self.h["off_data"]))
+        h = self.h
+
+        def arr(off, n, dt):
+            off = int(off)
+            return np.frombuffer(img[off: off + n * np.dtype(dt).itemsize].tobytes(), dtype=dt)
+
+        return {
+            "stats": arr(h["off_stats"], 8, "<u8"),
+            "state": arr(h["off_state"], self.nsb, "<u8"),
+            "next": arr(h["off_next"], self.nsb, "<u4"),
+            "bitmap": arr(h["off_bitmap"], self.nsb * self.bm_words, "<u4"),
+            "heads": arr(h["off_heads"], self.nctx * HEADS_PER_CTX, "<u8"),
+            "pool": arr(h["off_pool"], self.npool * POOL_STRIDE, "<u8").reshape(self.npool, POOL_STRIDE),
+            "pbitmap": arr(h["off_pbitmap"], (self.npages + 63) // 64, "<u8"),
+            "plen": arr(h["off_plen"], self.npages, "u1"),
+        }
+
+    def pool_members(self, m):
+        """Walk every pool shard; returns (list of superblocks, problems)."""
+        seen, probs = [], []
+        for s in range(self.npool):
+            top = int(m["pool"][s, 0]) & 0xFFFFFFFF
+            n = 0
+            while top:
+                seen.append(top - 1)
+                top = int(m["next"][top - 1])
+                n += 1
+                if n > self.nsb:
+                    probs.append("cycle in shard %d" % s)
+                    break
+            if n != int(m["pool"][s, 1]):
+                probs.append("shard %d count %d != walked %d" % (s, int(m["pool"][s, 1]), n))
+        return seen, probs
+
+    def canonical_problems(self):
+        """A heap with no live block, after trim: every bitmap and page-bitmap
+        word zero, every state word zero apart from its generation, every
+        context head empty, every superblock in exactly one pool shard."""
+        m = self.meta()
+        probs = []
+        if m["bitmap"].any():
+            probs.append("%d bitmap words nonzero" % int(np.count_nonzero(m["bitmap"])))
+        if m["pbitmap"].any():
+            probs.append("page bitmap nonzero")
+        gen_mask = np.uint64((1 << 40) - 1)
+        if (m["state"] & gen_mask).any():
+            probs.append("%d state words not free" % int(np.count_nonzero(m["state"] & gen_mask)))
+        if m["heads"].any():
+            probs.append("context heads not empty")
+        seen, p = self.pool_members(m)
+        probs += p
+        if sorted(seen) != list(range(self.nsb)):
+            probs.append("pool holds %d superblocks (%d distinct), expected %d"
+                         % (len(seen), len(set(seen)), self.nsb))
+        return probs
+
+
+def check_blocks(nh, offs, sizes):
+    """Invariants of live native blocks: in-heap, size-class-correct, aligned
+    to the block size, non-overlapping.  offs are heap offsets."""
+    probs = []
+    spans = []
+    data_end = nh.off_data + (nh.nsb << nh.sb_log2)
+    for off, size in zip(offs, sizes):
+        off, size = int(off), int(size)
+        if off == 0:
+            continue
+        if size <= nh.max_small:
+            cls = next(c for c in nh.classes if c >= size)
+            if not (nh.off_data <= off < data_end):
+                probs.append("block %#x outside superblock data" % off)
+                continue
+            inner = (off - nh.off_data) & ((1 << nh.sb_log2) - 1)
+            if inner % cls:
+                probs.append("block %#x not aligned to class %d" % (off, cls))
+            if inner + cls > (1 << nh.sb_log2):
+                probs.append("block %#x crosses its superblock" % off)
+            spans.append((off, off + cls))
+        else:
+            psz = 1 << nh.page_log2
+            if not (nh.off_pages <= off < nh.off_pages + (nh.npages << nh.page_log2)):
+                probs.append("large block %#x outside page region" % off)
+                continue
+            if (off - nh.off_pages) % psz:
+                probs.append("large block %#x not page aligned" % off)
+            spans.append((off, off + ((size + psz - 1) // psz) * psz))
+    spans.sort()
+    for a, b in zip(spans, spans[1:]):
+        if a[1] > b[0]:
+            probs.append("overlap %#x..%#x / %#x" % (a[0], a[1], b[0]))
+    return probs

@@ -1,0 +1,102 @@
+"""GPU tests of the native gfx950 layout (kma_device.h) through libkma.so.
+
+The native layout has no counterpart in the reference, so it is checked by
+invariants: every block in-heap, size-class-correct, aligned and
+non-overlapping; seeded patterns read back; and after a full malloc/free
+trace the metadata returns to its initial image apart from generation and
+ABA-tag fields and pool order."""
+import numpy as np
+import pytest
+
+import kma_amd
+from tests import natimg, refimg
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_init_is_canonical():
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 1024, npages=256) as h:
+        nh = natimg.NatHeap(h)
+        assert nh.classes == [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192]
+        assert nh.caps == [(1 << 14) // c for c in nh.classes]
+        assert nh.off_data % (1 << 14) == (-h.device_ptr) % (1 << 14)
+        assert nh.canonical_problems() == []
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_serial_trace_invariants(seed):
+    rng = np.random.default_rng(300 + seed)
+    sizes = [1, 7, 8, 9, 16, 24, 64, 100, 128, 500, 512, 1000, 2048, 4000, 8192, 8193,
+             16384, 20000, 65536]
+    ops, live = refimg.random_trace(rng, 4000, sizes, max_live=600)
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=4, npages=512) as h:
+        _say("native serial invariants seed=%d" % seed)
+        out = h.trace_serial(ops)
+        nh = natimg.NatHeap(h)
+        offs = [int(out[j]) for j in live]
+        sizes_live = [int(ops[j]) for j in live]
+        assert sum(1 for o, s in zip(offs, sizes_live) if o == 0 and s > 0) == 0
+        assert natimg.check_blocks(nh, offs, sizes_live) == []
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=4, npages=512) as h:
+        h.trace_serial(refimg.free_all(ops, live))
+        h.trim()
+        nh = natimg.NatHeap(h)
+        assert nh.canonical_problems() == []
+        assert int(nh.meta()["stats"][0]) == 0  # no double free
+
+
+def test_serial_size_zero_and_oversize():
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=1, npages=64) as h:
+        out = h.trace_serial([0, 8, -2, -1, (64 << 14) + 1, 1 << 20, -6])
+        assert out[0] == 0 and out[1] != 0 and out[4] == 0
+        assert out[5] != 0  # 64 pages of 16 KiB: the whole page region
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
+def test_double_free_detected_not_applied():
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=1) as h:
+        out = h.trace_serial([64, 64, -1, -1, -2])
+        assert out[0] and out[1]
+        nh = natimg.NatHeap(h)
+        assert int(nh.meta()["stats"][0]) == 1
+        h.trim()
+        assert nh.canonical_problems() == []
+
+
+def test_exhaustion_returns_null_then_recovers():
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 8, nctx=1, npool=1) as h:
+        # 8 superblocks of 2 x 8 KiB blocks
+        ops = [8192] * 20
+        frees = [-(i + 1) for i in range(20)]
+        out = h.trace_serial(ops + frees + [8192] * 16)
+        assert int(np.count_nonzero(out[:20])) == 16
+        assert int(np.count_nonzero(out[40:])) == 16
+
+
+def test_plain_small_native():
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 12, 4096) as h:
+        _say("native plain small")
+        st = h.run(kma_amd.WL_FIXED, 1024, 100, block=64, fixed=64)
+        assert st["ops"] == 204800 and st["nulls"] == 0 and st["bad"] == 0
+        assert st["double_frees"] == 0 and st["bad_frees"] == 0
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
+@pytest.mark.parametrize("form", [kma_amd.FORM_PLAIN, kma_amd.FORM_BATCHED])
+def test_full_occupancy_native(form):
+    """Full-occupancy workload (1 GiB heap, 16 KiB superblocks, 524,288
+    work-items, log-uniform 8 B..2 KiB, 0..64 holds), 3 rounds."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 65536) as h:
+        _say("native full occupancy form=%d" % form)
+        st = h.run(kma_amd.WL_LOGUNI, 524288, 3, form=form, block=256, seed=1)
+        assert st["ops"] == 2 * 3 * 524288
+        assert st["nulls"] == 0 and st["bad"] == 0
+        assert st["double_frees"] == 0 and st["bad_frees"] == 0
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
