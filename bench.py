@@ -75,7 +75,8 @@ def main():
 
     K, W = args.steps, args.warmup
     res = {}
-    for form, name in ((kma_amd.FORM_PLAIN, "plain"), (kma_amd.FORM_BATCHED, "batched")):
+    for form, name in ((kma_amd.FORM_PLAIN, "plain"), (kma_amd.FORM_FUSED, "fused"),
+                       (kma_amd.FORM_BATCHED, "batched")):
         with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS) as h:
             log("bench: %s form, %d warmup + %d timed rounds" % (name, W, K))
             st = h.run(kma_amd.WL_LOGUNI, NWI, K, form=form, block=BLOCK, seed=SEED, warmup=W)
@@ -124,8 +125,9 @@ def main():
             "form": "batched persistent (one launch, all rounds)",
         },
         "plain": {"value": p["ops_per_s"], "ms_per_step": p["gpu_ms"] / K, "launches": p["launches"]},
+        "fused_per_lane": {"value": res["fused"]["ops_per_s"], "ms_per_step": res["fused"]["gpu_ms"] / K},
         "batched_over_plain": value / p["ops_per_s"],
-        "invariants": {k: int(b[k] + p[k]) for k in ("nulls", "bad", "double_frees", "bad_frees")},
+        "invariants": {k: int(sum(r[k] for r in res.values())) for k in ("nulls", "bad", "double_frees", "bad_frees")},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,

@@ -77,8 +77,17 @@ typedef struct kma_dheap {
 #define KMA_STAT_DOUBLE_FREE 0
 #define KMA_STAT_BAD_FREE 1
 #define KMA_STAT_POOL_EMPTY 2
-#define KMA_STAT_SPIN_LIMIT 3
+#define KMA_STAT_WAIT_RESERVE 3 /* a reserve() wait hit the watchdog */
+#define KMA_STAT_WAIT_CLAIM 4   /* a per-lane bitmap claim hit the watchdog */
+#define KMA_STAT_WAIT_WGCLAIM 5 /* a wavefront bitmap claim hit the watchdog */
+#define KMA_STAT_WAIT_POOL 6    /* a pool push/pop hit the watchdog */
 #define KMA_NSTATS 8
+
+/* Every wait in the native allocator gives up after this many ticks of the
+ * constant 100 MHz wall clock (0.25 s).  Reaching it means the heap is
+ * corrupt or a caller broke the protocol; the call then fails and bumps its
+ * site's counter instead of spinning on. */
+#define KMA_WAIT_TICKS 25000000ull
 
 namespace kma_nat {
 
@@ -89,9 +98,23 @@ static constexpr uint64_t ST_CLASS_SHIFT = 32;
 static constexpr uint64_t ST_CLASS = 0x1full << ST_CLASS_SHIFT;
 static constexpr uint64_t ST_OWNED = 1ull << 37;
 static constexpr uint64_t ST_GEN_SHIFT = 40;
-/* Upper bound on retries of any wait; reaching it means the heap is corrupt:
- * the call counts KMA_STAT_SPIN_LIMIT and fails instead of spinning forever. */
-static constexpr uint32_t SPIN_LIMIT = 1u << 24;
+
+/* Watchdog for retry loops: reads the wall clock every 256 retries only. */
+struct waiter {
+	uint64_t t0 = 0;
+	uint32_t n = 0;
+	__device__ __forceinline__ bool expired()
+	{
+		if ((++n & 255u) != 0)
+			return false;
+		uint64_t t = wall_clock64();
+		if (t0 == 0) {
+			t0 = t;
+			return false;
+		}
+		return t - t0 > KMA_WAIT_TICKS;
+	}
+};
 
 __device__ __forceinline__ uint32_t st_count(uint64_t s) { return (uint32_t)(s & ST_COUNT); }
 __device__ __forceinline__ uint32_t st_cls1(uint64_t s) { return (uint32_t)((s & ST_CLASS) >> ST_CLASS_SHIFT); }
@@ -195,7 +218,12 @@ __device__ __forceinline__ uint32_t ctx_of(const kma_dheap *H)
 __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb)
 {
 	uint64_t *hp = pool(H, shard);
+	waiter wt;
 	for (;;) {
+		if (wt.expired()) {
+			bump_stat(H, KMA_STAT_WAIT_POOL);
+			return;
+		}
 		uint64_t h = ld_acq(hp);
 		st_rel(pnext(H, sb), (uint32_t)h);
 		uint64_t nh = ((h >> 32) + 1ull) << 32 | (uint64_t)(sb + 1u);
@@ -208,7 +236,12 @@ __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb
 __device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32_t *out)
 {
 	uint64_t *hp = pool(H, shard);
+	waiter wt;
 	for (;;) {
+		if (wt.expired()) {
+			bump_stat(H, KMA_STAT_WAIT_POOL);
+			return false;
+		}
 		uint64_t h = ld_acq(hp);
 		uint32_t top = (uint32_t)h;
 		if (top == 0)
@@ -272,9 +305,10 @@ __device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uin
 {
 	uint64_t *hp = head(H, ctx, c);
 	uint32_t cap = H->class_cap[c];
-	for (uint32_t spin = 0;; spin++) {
-		if (spin >= SPIN_LIMIT) {
-			bump_stat(H, KMA_STAT_SPIN_LIMIT);
+	waiter wt;
+	for (;;) {
+		if (wt.expired()) {
+			bump_stat(H, KMA_STAT_WAIT_RESERVE);
 			return resv{0, 0, 0, 0};
 		}
 		uint64_t h = ld_acq(hp);
@@ -333,7 +367,8 @@ __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint3
 	uint32_t b = start;
 	uint32_t lastw = (cap - 1u) >> 5;
 	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
-	for (uint32_t spin = 0; spin < SPIN_LIMIT; spin++) {
+	waiter wt;
+	while (!wt.expired()) {
 		uint32_t wi = b >> 5;
 		uint32_t w = ld_rlx(bm + wi);
 		uint32_t avail = ~w & (~0u << (b & 31u));
@@ -351,7 +386,7 @@ __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint3
 		if (b >= cap)
 			b = 0;
 	}
-	bump_stat(H, KMA_STAT_SPIN_LIMIT);
+	bump_stat(H, KMA_STAT_WAIT_CLAIM);
 	return nullptr;
 }
 

@@ -40,9 +40,26 @@ struct clheap; /* reference-layout heap header; accessed by offsets */
 namespace kma_ref {
 
 static constexpr uint64_t POISON = 1u;
-/* Upper bound on retries of any wait in this file; reaching it means the
- * heap is corrupt, and the call returns NULL instead of spinning forever. */
-static constexpr uint32_t SPIN_LIMIT = 1u << 24;
+/* Every wait in this file gives up after this many ticks of the constant
+ * 100 MHz wall clock (0.25 s); reaching it means the heap is corrupt, and
+ * the call returns NULL instead of spinning on. */
+static constexpr uint64_t WAIT_TICKS = 25000000ull;
+
+struct waiter {
+	uint64_t t0 = 0;
+	uint32_t n = 0;
+	__device__ __forceinline__ bool expired()
+	{
+		if ((++n & 255u) != 0)
+			return false;
+		uint64_t t = wall_clock64();
+		if (t0 == 0) {
+			t0 = t;
+			return false;
+		}
+		return t - t0 > WAIT_TICKS;
+	}
+};
 
 /* ---- agent-scope atomics ---- */
 template <typename T>
@@ -167,7 +184,10 @@ __device__ inline void q_enqueue(const view &v, char *item)
 	st(sb_next(item), q_ptr(0, tag - 1u));
 	uint32_t idx = ptr2idx(v, item);
 	uint32_t tailw;
+	waiter wt;
 	for (;;) {
+		if (wt.expired())
+			return;
 		tailw = ld(q_tail(v));
 		char *tail = idx2ptr(v, tailw);
 		uint32_t nextw = ld(sb_next(tail));
@@ -185,7 +205,10 @@ __device__ inline void q_enqueue(const view &v, char *item)
 
 __device__ inline char *q_dequeue(const view &v)
 {
+	waiter wt;
 	for (;;) {
+		if (wt.expired())
+			return nullptr;
 		uint32_t headw = ld(q_head(v));
 		char *head = idx2ptr(v, headw);
 		uint32_t tailw = ld(q_tail(v));
@@ -206,8 +229,9 @@ __device__ inline char *q_dequeue(const view &v)
 __device__ inline char *reserve(const view &v, int block, uint32_t *slot)
 {
 	uint64_t *hp = cls_head(v, block);
-	for (uint32_t spin = 0;; spin++) {
-		if (spin >= SPIN_LIMIT)
+	waiter wt;
+	for (;;) {
+		if (wt.expired())
 			return nullptr;
 		uint64_t cur = cas(hp, (uint64_t)0, POISON);
 		if (cur == 0) {
@@ -255,7 +279,8 @@ __device__ inline char *claim(const view &v, char *sb, uint32_t slot)
 	uint32_t slots = ld(sb_state(sb)) >> 16;
 	uint32_t bsize = ld(sb_size(sb));
 	uint32_t start = slot;
-	for (uint32_t pass = 0; pass < SPIN_LIMIT; pass++) {
+	waiter wt;
+	while (!wt.expired()) {
 		for (uint32_t b = slot; b < slots; b++) {
 			uint32_t *w = sb_word(v, sb, b);
 			uint32_t bit = 1u << (b & 31u);

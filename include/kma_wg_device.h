@@ -1,0 +1,259 @@
+/*
+ * kma_wg_device.h - work-group collective malloc/free on the native layout
+ * (HIP, gfx950).  Header-only.
+ *
+ * kma_wg_malloc / kma_wg_free are called by EVERY thread of a 1-D work-group
+ * (uniform control flow, like the reference's WG-collective
+ * clArrayList_grow_local, clArrayList.cl:111-176).  Each thread passes its own
+ * request (size 0 / NULL: none) and gets its own result; the work-group
+ * stages the batch in LDS so that the device-wide atomics are issued per
+ * size class and per bitmap word instead of per request:
+ *   malloc: LDS class histogram -> one reservation (atomicAdd on the
+ *           superblock state word) per class; one wavefront per class reads
+ *           the superblock bitmap coalesced (lane i <- word i), prefix-sums the
+ *           free bits and claims them with one atomicOr per word.
+ *   free:   an LDS hash keyed by (superblock, bitmap word) ORs the bits of all
+ *           requests that share a word; one atomicAnd per word and one
+ *           atomicSub on the state word per entry.
+ * Requests larger than the largest class go to the page allocator per lane.
+ */
+#ifndef KMA_WG_DEVICE_H
+#define KMA_WG_DEVICE_H
+
+#include "kma_device.h"
+
+template <int BLOCK>
+struct kma_wg_lds {
+	uint32_t cnt[32];
+	uint32_t cbase[33];
+	uint16_t order[BLOCK];
+	uint64_t res[BLOCK];
+	unsigned long long hkey[2 * BLOCK];
+	uint32_t hmask[2 * BLOCK];
+	uint32_t hn[2 * BLOCK];
+};
+
+namespace kma_nat {
+
+/* Exclusive prefix sum over a full wavefront (all 64 lanes active). */
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total)
+{
+	uint32_t x = v;
+	for (int o = 1; o < 64; o <<= 1) {
+		uint32_t y = __shfl_up(x, o);
+		if ((int)lane() >= o)
+			x += y;
+	}
+	*total = __shfl(x, 63);
+	return x - v;
+}
+
+/* The lowest `k` set bits of `m`. */
+__device__ __forceinline__ uint32_t low_bits(uint32_t m, uint32_t k)
+{
+	uint32_t r = 0;
+	for (uint32_t i = 0; i < k; i++) {
+		uint32_t b = m & (0u - m);
+		r |= b;
+		m ^= b;
+	}
+	return r;
+}
+
+/* One full wavefront claims `got` free bits of superblock sb (class c),
+ * starting its scan at the word holding bit `base`, and writes the block
+ * offsets (from the heap base) into res[order[0..got-1]]. */
+__device__ inline uint32_t wave_claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t base, uint32_t got,
+				      const uint16_t *order, uint64_t *res)
+{
+	uint32_t cap = H->class_cap[c];
+	uint32_t W = (cap + 31u) >> 5;
+	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
+	uint32_t *bm = bitmap(H, sb);
+	uint64_t d0 = H->off_data + ((uint64_t)sb << H->sb_log2);
+	uint32_t bsize = H->class_size[c];
+	uint32_t done = 0;
+	uint32_t w0 = (base >> 5) % W;
+	waiter wt;
+	while (done < got && !wt.expired()) {
+		for (uint32_t chunk = 0; chunk < W && done < got; chunk += 64) {
+			uint32_t k = chunk + lane();
+			bool has = k < W;
+			uint32_t wi = has ? (w0 + k) % W : 0;
+			uint32_t freeb = 0;
+			if (has) {
+				freeb = ~ld_rlx(bm + wi);
+				if (wi == W - 1)
+					freeb &= tail;
+			}
+			uint32_t tot;
+			uint32_t pre = wave_excl_scan(__popc(freeb), &tot);
+			uint32_t need = got - done;
+			uint32_t take = pre >= need ? 0u : min((uint32_t)__popc(freeb), need - pre);
+			uint32_t mask = low_bits(freeb, take);
+			uint32_t won = 0;
+			if (mask)
+				won = mask & ~f_or(bm + wi, mask);
+			uint32_t wtot;
+			uint32_t wpre = wave_excl_scan(__popc(won), &wtot);
+			uint32_t r = done + wpre;
+			while (won) {
+				uint32_t b = __ffs(won) - 1u;
+				won &= won - 1u;
+				res[order[r++]] = d0 + (uint64_t)(wi * 32u + b) * bsize;
+			}
+			done += wtot;
+		}
+	}
+	if (done < got && lane() == 0)
+		bump_stat(H, KMA_STAT_WAIT_WGCLAIM);
+	return done;
+}
+
+__device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
+{
+	return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (n - 1u);
+}
+
+} /* namespace kma_nat */
+
+/* Work-group collective malloc.  All threads of the 1-D work-group call it;
+ * size 0 means "no request" for that thread.  `ctx` is the allocation
+ * context the work-group reserves from. */
+template <int BLOCK>
+__device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t ctx)
+{
+	using namespace kma_nat;
+	const uint32_t tid = threadIdx.x, nth = blockDim.x, nw = (nth + 63u) >> 6, w = tid >> 6;
+	const uint32_t ncls = H->nclasses;
+	if (tid < 32)
+		L.cnt[tid] = 0;
+	__syncthreads();
+	int c = -1;
+	if (size > 0 && size <= H->max_small)
+		c = class_of(H, (uint32_t)size);
+	uint32_t q = 0;
+	if (c >= 0)
+		q = atomicAdd(&L.cnt[c], 1u);
+	__syncthreads();
+	if (tid == 0) {
+		uint32_t acc = 0;
+		for (uint32_t k = 0; k < ncls; k++) {
+			L.cbase[k] = acc;
+			acc += L.cnt[k];
+		}
+		L.cbase[ncls] = acc;
+	}
+	__syncthreads();
+	if (c >= 0)
+		L.order[L.cbase[c] + q] = (uint16_t)tid;
+	__syncthreads();
+	/* one wavefront per class: reserve a segment (lane 0), claim it (all
+	 * lanes), until the class is served or the heap cannot serve it */
+	for (uint32_t k = w; k < ncls; k += nw) {
+		uint32_t n = L.cnt[k];
+		if (n == 0)
+			continue;
+		const uint16_t *ord = &L.order[L.cbase[k]];
+		uint32_t r0 = 0;
+		while (r0 < n) {
+			resv r = {0, 0, 0, 0};
+			if (lane() == 0)
+				r = reserve(H, ctx, k, n - r0);
+			uint32_t ok = rdlane(r.ok, 0);
+			if (!ok)
+				break;
+			uint32_t sb = rdlane(r.sb, 0), rb = rdlane(r.base, 0), got = rdlane(r.got, 0);
+			uint32_t done = wave_claim(H, sb, k, rb, got, ord + r0, L.res);
+			r0 += done;
+			if (done < got) {
+				/* give back what could not be claimed */
+				if (lane() == 0)
+					undo(H, sb, got - done, ctx);
+				break;
+			}
+		}
+		for (uint32_t r = r0 + lane(); r < n; r += 64)
+			L.res[ord[r]] = 0;
+	}
+	__syncthreads();
+	void *out = nullptr;
+	if (c >= 0) {
+		uint64_t off = L.res[tid];
+		out = off ? (void *)(base(H) + off) : nullptr;
+	} else if (size > H->max_small) {
+		out = page_alloc(H, size, (blockIdx.x * nth + tid) ^ (uint32_t)clock64());
+	}
+	__syncthreads();
+	return out;
+}
+
+/* Work-group collective free.  All threads call it; NULL means "no
+ * request".  Double frees are detected and counted, not applied. */
+template <int BLOCK>
+__device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L, uint32_t ctx)
+{
+	using namespace kma_nat;
+	const uint32_t tid = threadIdx.x, nth = blockDim.x;
+	const uint32_t HN = 2u * BLOCK;
+	for (uint32_t i = tid; i < HN; i += nth) {
+		L.hkey[i] = 0;
+		L.hmask[i] = 0;
+		L.hn[i] = 0;
+	}
+	__syncthreads();
+	bool owner = false;
+	uint32_t slot = 0;
+	if (ptr) {
+		char *p = (char *)ptr;
+		char *d0 = base(H) + H->off_data;
+		uint64_t off = (uint64_t)(p - d0);
+		if (p >= d0 && off < (H->nsb << H->sb_log2)) {
+			uint32_t sb = (uint32_t)(off >> H->sb_log2);
+			uint32_t c1 = st_cls1(ld_rlx(state(H, sb)));
+			if (c1 == 0) {
+				bump_stat(H, KMA_STAT_BAD_FREE);
+			} else {
+				uint64_t inoff = off & ((1ull << H->sb_log2) - 1ull);
+				uint32_t idx = (uint32_t)((inoff * H->class_magic[c1 - 1u]) >> 40);
+				unsigned long long key = (unsigned long long)sb * H->bm_words + (idx >> 5) + 1ull;
+				uint32_t h = wg_hash(key, HN);
+				for (;;) {
+					unsigned long long old = atomicCAS(&L.hkey[h], 0ull, key);
+					if (old == 0ull || old == key) {
+						owner = old == 0ull;
+						slot = h;
+						break;
+					}
+					h = (h + 1u) & (HN - 1u);
+				}
+				atomicOr(&L.hmask[slot], 1u << (idx & 31u));
+				atomicAdd(&L.hn[slot], 1u);
+			}
+		} else if (H->npages && p >= base(H) + H->off_pages) {
+			page_free(H, p);
+		} else {
+			bump_stat(H, KMA_STAT_BAD_FREE);
+		}
+	}
+	__syncthreads();
+	if (owner) {
+		unsigned long long key = L.hkey[slot] - 1ull;
+		uint32_t sb = (uint32_t)(key / H->bm_words);
+		uint32_t wi = (uint32_t)(key % H->bm_words);
+		uint32_t mask = L.hmask[slot];
+		uint32_t old = f_and(bitmap(H, sb) + wi, ~mask);
+		uint32_t n = __popc(old & mask);
+		uint32_t dbl = L.hn[slot] - n;
+		if (dbl)
+			fadd(stats(H) + KMA_STAT_DOUBLE_FREE, (uint64_t)dbl);
+		if (n) {
+			uint64_t ns = fsub(state(H, sb), (uint64_t)n) - n;
+			if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
+				try_release(H, sb, ns, ctx);
+		}
+	}
+	__syncthreads();
+}
+
+#endif /* KMA_WG_DEVICE_H */

@@ -23,8 +23,10 @@ extern "C" {
 #define KMA_WL_HIGHVAR  4u /* kma_test_malloc_highvar, kma.cl:494-526 */
 
 /* Execution forms. */
-#define KMA_FORM_PLAIN   0u /* one kernel launch per round (tb_kma.c:263) */
-#define KMA_FORM_BATCHED 1u /* one graph-captured persistent launch, all rounds */
+#define KMA_FORM_PLAIN   0u /* one kernel launch per round (tb_kma.c:263), per-lane API */
+#define KMA_FORM_BATCHED 1u /* one graph-captured persistent launch, all rounds,
+			       work-group collective API (native layout only) */
+#define KMA_FORM_FUSED   2u /* one launch, all rounds, per-lane API */
 
 typedef struct kma_workload {
 	uint32_t mode;    /* KMA_WL_* */
@@ -49,6 +51,8 @@ typedef struct kma_run_stats {
 	uint64_t bad;           /* pattern read-back mismatches */
 	uint64_t double_frees;  /* heap counter (native layout) */
 	uint64_t bad_frees;     /* heap counter (native layout) */
+	uint64_t pool_empty;    /* refills that found no free superblock */
+	uint64_t waits[4];      /* watchdog expiries: reserve, claim, wavefront claim, pool */
 } kma_run_stats;
 
 /* Run a per-work-item workload on the heap. */

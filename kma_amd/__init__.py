@@ -18,7 +18,7 @@ LAYOUT_NATIVE = 1
 MAX_CLASSES = 24
 
 WL_FIXED, WL_LOGUNI, WL_REF_TEST, WL_LOWVAR, WL_HIGHVAR = range(5)
-FORM_PLAIN, FORM_BATCHED = 0, 1
+FORM_PLAIN, FORM_BATCHED, FORM_FUSED = 0, 1, 2
 
 # Exported symbols, as declared in include/*.h (checked by the CPU tests).
 EXPORTS = [
@@ -69,10 +69,14 @@ class RunStats(ctypes.Structure):
         ("bad", ctypes.c_uint64),
         ("double_frees", ctypes.c_uint64),
         ("bad_frees", ctypes.c_uint64),
+        ("pool_empty", ctypes.c_uint64),
+        ("waits", ctypes.c_uint64 * 4),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["waits"] = list(self.waits)
+        return d
 
 
 class KmaError(RuntimeError):

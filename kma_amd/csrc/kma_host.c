@@ -461,7 +461,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		return -1;
 	}
 	memset(out, 0, sizeof(*out));
-	if (wl->mode > KMA_WL_HIGHVAR || wl->nwi == 0 || wl->block == 0 || (wl->block & 63) || wl->block > 256) {
+	if (wl->mode > KMA_WL_HIGHVAR || wl->form > KMA_FORM_FUSED || wl->nwi == 0 || wl->block == 0 || (wl->block & 63) || wl->block > 256) {
 		set_err("bad workload");
 		return -1;
 	}
@@ -472,6 +472,8 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	int rc = -1;
 	uint64_t st0[NAT_NSTATS], st1[NAT_NSTATS];
 	unsigned long long hs[4];
+	hipGraph_t graph = NULL;
+	hipGraphExec_t gexec = NULL;
 	if (read_native_stats(h, st0) != 0)
 		return -1;
 	if (hipMalloc((void **)&d_stats, 4 * 8) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
@@ -479,12 +481,20 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		set_err("workload setup failed");
 		goto done;
 	}
+	if (wl->form == KMA_FORM_BATCHED && layout != 1) {
+		set_err("the batched form needs a native-layout heap");
+		goto done;
+	}
 	/* warm-up rounds, untimed, not counted */
 	if (hipMemset(d_stats, 0, 32) != hipSuccess)
 		goto done;
-	for (uint32_t r = 0; r < wl->warmup; r++) {
-		rc = kmad_rounds(layout, h->dev, wl->round0 + r, 1, (int)wl->mode, wl->fixed, wl->seed, wl->nwi,
-				 wl->block, d_stats, s);
+	if (wl->warmup) {
+		if (wl->form == KMA_FORM_BATCHED)
+			rc = kmad_batched(h->dev, wl->round0, wl->warmup, (int)wl->mode, wl->fixed, wl->seed, wl->nwi,
+					  d_stats, s);
+		else
+			rc = kmad_rounds(layout, h->dev, wl->round0, wl->warmup, (int)wl->mode, wl->fixed, wl->seed,
+					 wl->nwi, wl->block, d_stats, s);
 		if (rc) {
 			set_err("warm-up launch failed: %d", rc);
 			goto done;
@@ -500,6 +510,21 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		goto done;
 	}
 	uint32_t r0 = wl->round0 + wl->warmup;
+	if (wl->form == KMA_FORM_BATCHED) {
+		/* one persistent launch for all rounds, captured in a graph */
+		if (hipStreamBeginCapture(s, hipStreamCaptureModeGlobal) != hipSuccess) {
+			set_err("graph capture failed");
+			rc = -1;
+			goto done;
+		}
+		rc = kmad_batched(h->dev, r0, wl->rounds, (int)wl->mode, wl->fixed, wl->seed, wl->nwi, d_stats, s);
+		if (hipStreamEndCapture(s, &graph) != hipSuccess || rc != 0 ||
+		    hipGraphInstantiate(&gexec, graph, NULL, NULL, 0) != hipSuccess) {
+			set_err("graph build failed: %d", rc);
+			rc = -1;
+			goto done;
+		}
+	}
 	hipEventRecord(e0, s);
 	if (wl->form == KMA_FORM_PLAIN) {
 		for (uint32_t r = 0; r < wl->rounds; r++) {
@@ -511,6 +536,13 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 			}
 		}
 		out->launches = wl->rounds;
+	} else if (wl->form == KMA_FORM_BATCHED) {
+		if (hipGraphLaunch(gexec, s) != hipSuccess) {
+			set_err("graph launch failed");
+			rc = -1;
+			goto done;
+		}
+		out->launches = 1;
 	} else {
 		rc = kmad_rounds(layout, h->dev, r0, wl->rounds, (int)wl->mode, wl->fixed, wl->seed, wl->nwi,
 				 wl->block, d_stats, s);
@@ -541,8 +573,15 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	out->mallocs = hs[3];
 	out->double_frees = st1[NAT_STAT_DOUBLE_FREE] - st0[NAT_STAT_DOUBLE_FREE];
 	out->bad_frees = st1[NAT_STAT_BAD_FREE] - st0[NAT_STAT_BAD_FREE];
+	out->pool_empty = st1[2] - st0[2];
+	for (int k = 0; k < 4; k++)
+		out->waits[k] = st1[3 + k] - st0[3 + k];
 	rc = 0;
 done:
+	if (gexec)
+		hipGraphExecDestroy(gexec);
+	if (graph)
+		hipGraphDestroy(graph);
 	if (e0)
 		hipEventDestroy(e0);
 	if (e1)

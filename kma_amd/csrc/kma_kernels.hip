@@ -7,6 +7,7 @@
 #include "kma_device.h"
 #include "kma_launch.h"
 #include "kma_ref_device.h"
+#include "kma_wg_device.h"
 #include "kma_work.h"
 
 #define KMA_CHECK(x)                                   \
@@ -350,6 +351,74 @@ extern "C" int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, u
 	uint64_t blocks = (nthreads + block - 1) / block;
 	hipLaunchKernelGGL(k_atomic_peak, dim3((uint32_t)blocks), dim3(block), 0, (hipStream_t)stream,
 			   (uint32_t *)buf, nwords, iters, pattern);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* batched persistent form                                              */
+/* ------------------------------------------------------------------ */
+/*
+ * One launch runs every round; each work-group stages its requests in LDS
+ * and allocates / frees them collectively (kma_wg_device.h).  Work-items
+ * past nwi take part with empty requests so control flow stays uniform.
+ */
+#define KMA_BATCH_BLOCK 256
+
+__global__ void __launch_bounds__(KMA_BATCH_BLOCK) k_batched(kma_dheap *H, uint32_t round0, uint32_t nrounds, int mode,
+							     uint32_t fixed, uint64_t seed, uint64_t nwi,
+							     unsigned long long *stats)
+{
+	__shared__ kma_wg_lds<KMA_BATCH_BLOCK> L;
+	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	bool active = gid < nwi;
+	uint32_t ctx = blockIdx.x & (H->nctx - 1u);
+	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
+	for (uint32_t r = round0; r < round0 + nrounds; r++) {
+		uint32_t size = 0, hold = 0;
+		if (active) {
+			if (mode == KMA_WL_FIXED) {
+				size = fixed;
+			} else {
+				size = 1u << (3 + kma_mix(seed, gid, r, 0) % 9);
+				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
+			}
+		}
+		volatile uint32_t *p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, ctx);
+		if (active) {
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+			} else {
+				uint32_t pat = (uint32_t)(gid << 8) ^ r;
+				if (mode == KMA_WL_FIXED) {
+					p[0] = (uint32_t)gid;
+					p[1] = r;
+					if (p[0] != (uint32_t)gid || p[1] != r)
+						bad++;
+				} else {
+					uint32_t words = size >> 2;
+					p[0] = pat;
+					for (uint32_t k = 0; k < hold; k++)
+						p[1 + k % (words - 1)] = pat + k + 1;
+					if (p[0] != pat)
+						bad++;
+				}
+				ops++;
+			}
+		}
+		kma_wg_free<KMA_BATCH_BLOCK>(H, (void *)p, L, ctx);
+	}
+	wave_add_stats(stats, ops, nulls, bad, mallocs);
+}
+
+extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed, uint64_t seed,
+			    uint64_t nwi, unsigned long long *d_stats, void *stream)
+{
+	uint64_t blocks = (nwi + KMA_BATCH_BLOCK - 1) / KMA_BATCH_BLOCK;
+	hipLaunchKernelGGL(k_batched, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, (hipStream_t)stream,
+			   (kma_dheap *)heap, round0, nrounds, mode, fixed, seed, nwi, d_stats);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }

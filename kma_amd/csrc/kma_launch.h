@@ -28,6 +28,10 @@ int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds,
 		int mode, uint32_t fixed, uint64_t seed, uint64_t nwi,
 		uint32_t block, unsigned long long *d_stats, void *stream);
 
+int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode,
+		 uint32_t fixed, uint64_t seed, uint64_t nwi,
+		 unsigned long long *d_stats, void *stream);
+
 int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses);
 
 int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, uint32_t iters,

@@ -15,4 +15,4 @@ def pytest_configure(config):
 def pytest_collection_modifyitems(config, items):
     for it in items:
         if "gpu" in it.keywords:
-            it.add_marker(pytest.mark.timeout(300))
+            it.add_marker(pytest.mark.timeout(90))

@@ -88,15 +88,36 @@ def test_plain_small_native():
         assert natimg.NatHeap(h).canonical_problems() == []
 
 
-@pytest.mark.parametrize("form", [kma_amd.FORM_PLAIN, kma_amd.FORM_BATCHED])
+@pytest.mark.parametrize("form", [kma_amd.FORM_PLAIN, kma_amd.FORM_FUSED, kma_amd.FORM_BATCHED])
 def test_full_occupancy_native(form):
     """Full-occupancy workload (1 GiB heap, 16 KiB superblocks, 524,288
     work-items, log-uniform 8 B..2 KiB, 0..64 holds), 3 rounds."""
     with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 65536) as h:
         _say("native full occupancy form=%d" % form)
         st = h.run(kma_amd.WL_LOGUNI, 524288, 3, form=form, block=256, seed=1)
-        assert st["ops"] == 2 * 3 * 524288
-        assert st["nulls"] == 0 and st["bad"] == 0
+        assert st["ops"] == 2 * 3 * 524288, st
+        assert st["nulls"] == 0 and st["bad"] == 0 and st["waits"] == [0, 0, 0, 0], st
         assert st["double_frees"] == 0 and st["bad_frees"] == 0
         h.trim()
         assert natimg.NatHeap(h).canonical_problems() == []
+
+
+def test_batched_plain_small_and_partial_workgroup():
+    """Batched form with a work-item count that leaves the last work-group
+    partly empty, and the fixed-size workload."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 12, 4096) as h:
+        _say("native batched fixed 1000 wi")
+        st = h.run(kma_amd.WL_FIXED, 1000, 50, form=kma_amd.FORM_BATCHED, block=256, fixed=64)
+        assert st["ops"] == 2 * 50 * 1000 and st["nulls"] == 0 and st["bad"] == 0, st
+        _say("native batched loguni 777 wi")
+        st = h.run(kma_amd.WL_LOGUNI, 777, 20, form=kma_amd.FORM_BATCHED, block=256, seed=5)
+        assert st["ops"] == 2 * 20 * 777 and st["nulls"] == 0 and st["bad"] == 0, st
+        assert st["double_frees"] == 0 and st["bad_frees"] == 0
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
+def test_batched_rejects_reference_layout():
+    with kma_amd.Heap(kma_amd.LAYOUT_REFERENCE, 12, 64) as h:
+        with pytest.raises(kma_amd.KmaError):
+            h.run(kma_amd.WL_FIXED, 64, 1, form=kma_amd.FORM_BATCHED, block=256)
