@@ -82,8 +82,9 @@ def main():
             st = h.run(kma_amd.WL_LOGUNI, NWI, K, form=form, block=BLOCK, seed=SEED, warmup=W)
             st["ops_per_s"] = st["ops"] / (st["gpu_ms"] * 1e-3)
             res[name] = st
-            log("bench: %s %.3e ops/s, %.3f ms/round, nulls %d bad %d"
-                % (name, st["ops_per_s"], st["gpu_ms"] / K, st["nulls"], st["bad"]))
+            log("bench: %s %.3e ops/s, %.3f ms/round, nulls %d bad %d dbl %d badfree %d pool_empty %d waits %s"
+                % (name, st["ops_per_s"], st["gpu_ms"] / K, st["nulls"], st["bad"], st["double_frees"],
+                   st["bad_frees"], st["pool_empty"], st["waits"]))
 
     import ctypes
 

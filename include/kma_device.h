@@ -306,6 +306,7 @@ __device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uin
 	uint64_t *hp = head(H, ctx, c);
 	uint32_t cap = H->class_cap[c];
 	waiter wt;
+	uint32_t bo = 1;
 	for (;;) {
 		if (wt.expired()) {
 			bump_stat(H, KMA_STAT_WAIT_RESERVE);
@@ -333,7 +334,9 @@ __device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uin
 			return resv{sb, cnt, got, 1};
 		}
 		if (h == HEAD_BUSY) {
-			__builtin_amdgcn_s_sleep(2);
+			for (uint32_t i = 0; i < bo; i++)
+				__builtin_amdgcn_s_sleep(4);
+			bo = bo < 64u ? 2u * bo : 64u;
 			continue;
 		}
 		uint32_t sb = (uint32_t)h - 2u;
@@ -468,7 +471,7 @@ __device__ inline void *kma_malloc(kma_dheap *H, size_t size)
 	if (size == 0)
 		return nullptr;
 	if (size > H->max_small) {
-		uint32_t seed = (blockIdx.x * blockDim.x + threadIdx.x) ^ (uint32_t)__builtin_amdgcn_s_memtime();
+		uint32_t seed = (blockIdx.x * blockDim.x + threadIdx.x) ^ (uint32_t)clock64();
 		return page_alloc(H, size, seed);
 	}
 	uint32_t c = (uint32_t)class_of(H, (uint32_t)size);

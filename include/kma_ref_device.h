@@ -41,9 +41,9 @@ namespace kma_ref {
 
 static constexpr uint64_t POISON = 1u;
 /* Every wait in this file gives up after this many ticks of the constant
- * 100 MHz wall clock (0.25 s); reaching it means the heap is corrupt, and
+ * 100 MHz wall clock; reaching it means the heap is corrupt or starved, and
  * the call returns NULL instead of spinning on. */
-static constexpr uint64_t WAIT_TICKS = 25000000ull;
+static constexpr uint64_t WAIT_TICKS = 400000000ull; /* 4 s */
 
 struct waiter {
 	uint64_t t0 = 0;
@@ -83,6 +83,16 @@ template <typename T>
 __device__ __forceinline__ T xchg(T *p, T v)
 {
 	return __hip_atomic_exchange(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Exponential back-off between retries of a spin on a shared word, so that
+ * thousands of spinning lanes do not starve the lane they wait for of the
+ * word's atomic unit. */
+__device__ __forceinline__ void backoff(uint32_t &k)
+{
+	for (uint32_t i = 0; i < k; i++)
+		__builtin_amdgcn_s_sleep(4);
+	k = k < 64u ? 2u * k : 64u;
 }
 
 /* ---- layout ---- */
@@ -230,6 +240,7 @@ __device__ inline char *reserve(const view &v, int block, uint32_t *slot)
 {
 	uint64_t *hp = cls_head(v, block);
 	waiter wt;
+	uint32_t bo = 1;
 	for (;;) {
 		if (wt.expired())
 			return nullptr;
@@ -259,7 +270,7 @@ __device__ inline char *reserve(const view &v, int block, uint32_t *slot)
 				 * rather than wait (see note 3 above). */
 				if ((so >> 16) == 1u)
 					cas(hp, cur, (uint64_t)0);
-				__builtin_amdgcn_s_sleep(1);
+				backoff(bo);
 				continue;
 			}
 			fr--;
@@ -270,7 +281,7 @@ __device__ inline char *reserve(const view &v, int block, uint32_t *slot)
 			*slot = fr;
 			return sb;
 		}
-		__builtin_amdgcn_s_sleep(1);
+		backoff(bo);
 	}
 }
 

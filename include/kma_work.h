@@ -37,8 +37,10 @@ typedef struct kma_workload {
 	uint32_t round0;  /* first round index (trace position) */
 	uint32_t fixed;   /* size for KMA_WL_FIXED */
 	uint64_t seed;
-	uint32_t warmup;  /* untimed rounds run first (PLAIN) / untimed launches (BATCHED) */
-	uint32_t reps;    /* timed repetitions (BATCHED: graph launches) */
+	uint32_t warmup;  /* untimed rounds run first */
+	uint32_t reps;    /* reserved, 0 */
+	uint32_t variant; /* BATCHED only: bit 0 per-lane kma_malloc, bit 1 per-lane
+			     kma_free, in place of the work-group collective calls */
 } kma_workload;
 
 typedef struct kma_run_stats {

@@ -55,6 +55,7 @@ class Workload(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("warmup", ctypes.c_uint32),
         ("reps", ctypes.c_uint32),
+        ("variant", ctypes.c_uint32),
     ]
 
 
@@ -210,8 +211,8 @@ class Heap:
         return out
 
     def run(self, mode, nwi, rounds, form=FORM_PLAIN, block=64, fixed=64, seed=1,
-            round0=0, warmup=0, reps=1):
-        w = Workload(mode, form, nwi, block, rounds, round0, fixed, seed, warmup, reps)
+            round0=0, warmup=0, variant=0):
+        w = Workload(mode, form, nwi, block, rounds, round0, fixed, seed, warmup, 0, variant)
         st = RunStats()
         _check(lib().kma_run_workload(self._h, ctypes.byref(w), ctypes.byref(st)), "kma_run_workload")
         return st.as_dict()

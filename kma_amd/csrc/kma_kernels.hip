@@ -9,6 +9,7 @@
 #include "kma_ref_device.h"
 #include "kma_wg_device.h"
 #include "kma_work.h"
+#include "kma_workloads.h"
 
 #define KMA_CHECK(x)                                   \
 	do {                                           \
@@ -16,19 +17,6 @@
 		if (e_ != hipSuccess)                  \
 			return -(int)e_;               \
 	} while (0)
-
-/* ------------------------------------------------------------------ */
-/* deterministic trace values (same function as the host side)         */
-/* ------------------------------------------------------------------ */
-__device__ __forceinline__ uint64_t kma_mix(uint64_t seed, uint64_t gid, uint64_t round, uint64_t k)
-{
-	uint64_t z = seed * 0x9E3779B97F4A7C15ull + gid * 0xBF58476D1CE4E5B9ull +
-		     round * 0x94D049BB133111EBull + k * 0xD6E8FEB86659FD93ull;
-	z += 0x9E3779B97F4A7C15ull;
-	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-	return z ^ (z >> 31);
-}
 
 /* ------------------------------------------------------------------ */
 /* reference layout: parallel heap init                                 */
@@ -182,120 +170,12 @@ extern "C" int kmad_serial(int layout, void *heap, const int64_t *d_ops, uint64_
 /* ------------------------------------------------------------------ */
 /* plain per-launch workloads                                           */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ void wave_add_stats(unsigned long long *stats, uint64_t a, uint64_t b, uint64_t c,
-					       uint64_t d)
-{
-	/* one atomic per counter per wavefront */
-	uint64_t v[4] = {a, b, c, d};
-	for (int k = 0; k < 4; k++) {
-		uint64_t x = v[k];
-		for (int off = 32; off > 0; off >>= 1)
-			x += __shfl_xor(x, off);
-		if (__lane_id() == 0 && x)
-			atomicAdd(stats + k, (unsigned long long)x);
-	}
-}
-
-/*
- * One kernel body for every per-work-item workload:
- *  KMA_WL_FIXED    : malloc(fixed), write {gid, round}, read back, free
- *                    (plain small workload)
- *  KMA_WL_LOGUNI   : malloc(2^e), e uniform in 3..11, hold 0..64 writes,
- *                    read back, free (full-occupancy workload)
- *  KMA_WL_REF_TEST : kma_test_malloc (kma.cl:424-461): malloc(8), write pid,
- *                    free; a NULL ends the work-item
- *  KMA_WL_LOWVAR   : kma_test_malloc_lowvar (kma.cl:463-492)
- *  KMA_WL_HIGHVAR  : kma_test_malloc_highvar (kma.cl:494-526)
- * stats: [0] heap ops issued, [1] NULL returns, [2] read-back mismatches,
- *        [3] mallocs issued.
- */
 template <class HeapT>
 __global__ void __launch_bounds__(256) k_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds, int mode,
 						uint32_t fixed, uint64_t seed, uint64_t nwi,
 						unsigned long long *stats)
 {
-	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
-	bool alive = gid < nwi;
-	for (uint32_t r = round0; r < round0 + nrounds; r++) {
-		if (!alive)
-			break;
-		if (mode == KMA_WL_FIXED || mode == KMA_WL_LOGUNI) {
-			uint32_t size, hold = 0;
-			if (mode == KMA_WL_FIXED) {
-				size = fixed;
-			} else {
-				size = 1u << (3 + kma_mix(seed, gid, r, 0) % 9);
-				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
-			}
-			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, size);
-			mallocs++;
-			ops++;
-			if (!p) {
-				nulls++;
-				continue;
-			}
-			uint32_t pat = (uint32_t)(gid << 8) ^ r;
-			if (mode == KMA_WL_FIXED) {
-				p[0] = (uint32_t)gid;
-				p[1] = r;
-				if (p[0] != (uint32_t)gid || p[1] != r)
-					bad++;
-			} else {
-				uint32_t words = size >> 2;
-				p[0] = pat;
-				for (uint32_t k = 0; k < hold; k++)
-					p[1 + k % (words - 1)] = pat + k + 1;
-				if (p[0] != pat)
-					bad++;
-			}
-			kma_free(heap, (void *)p);
-			ops++;
-		} else if (mode == KMA_WL_REF_TEST) {
-			volatile uint64_t *p = (volatile uint64_t *)kma_malloc(heap, 8);
-			mallocs++;
-			ops++;
-			if (!p) {
-				nulls++;
-				alive = false;
-				continue;
-			}
-			p[0] = gid;
-			kma_free(heap, (void *)p);
-			ops++;
-		} else {
-			uint32_t amount;
-			if (mode == KMA_WL_LOWVAR)
-				amount = (gid & 1) ? 8 : 16;
-			else
-				amount = 4u << ((gid + r) % 5);
-			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, amount);
-			mallocs++;
-			ops++;
-			if (!p) {
-				nulls++;
-				alive = false;
-				continue;
-			}
-			p[0] = mode == KMA_WL_LOWVAR ? (uint32_t)gid : 4919u;
-			if (gid == 0) {
-				volatile uint32_t *q = (volatile uint32_t *)kma_malloc(heap, 4000);
-				mallocs++;
-				ops++;
-				if (!q) {
-					nulls++;
-					alive = false;
-					continue;
-				}
-				q[8] = 4919;
-				kma_free(heap, (void *)q);
-				ops++;
-			}
-			kma_free(heap, (void *)p);
-			ops++;
-		}
-	}
-	wave_add_stats(stats, ops, nulls, bad, mallocs);
+	wl_rounds<HeapT>(heap, round0, nrounds, mode, fixed, seed, nwi, stats);
 }
 
 extern "C" int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed,
@@ -363,62 +243,20 @@ extern "C" int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, u
  * and allocates / frees them collectively (kma_wg_device.h).  Work-items
  * past nwi take part with empty requests so control flow stays uniform.
  */
-#define KMA_BATCH_BLOCK 256
-
 __global__ void __launch_bounds__(KMA_BATCH_BLOCK) k_batched(kma_dheap *H, uint32_t round0, uint32_t nrounds, int mode,
 							     uint32_t fixed, uint64_t seed, uint64_t nwi,
-							     unsigned long long *stats)
+							     uint32_t variant, unsigned long long *stats)
 {
 	__shared__ kma_wg_lds<KMA_BATCH_BLOCK> L;
-	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	bool active = gid < nwi;
-	uint32_t ctx = blockIdx.x & (H->nctx - 1u);
-	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
-	for (uint32_t r = round0; r < round0 + nrounds; r++) {
-		uint32_t size = 0, hold = 0;
-		if (active) {
-			if (mode == KMA_WL_FIXED) {
-				size = fixed;
-			} else {
-				size = 1u << (3 + kma_mix(seed, gid, r, 0) % 9);
-				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
-			}
-		}
-		volatile uint32_t *p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, ctx);
-		if (active) {
-			mallocs++;
-			ops++;
-			if (!p) {
-				nulls++;
-			} else {
-				uint32_t pat = (uint32_t)(gid << 8) ^ r;
-				if (mode == KMA_WL_FIXED) {
-					p[0] = (uint32_t)gid;
-					p[1] = r;
-					if (p[0] != (uint32_t)gid || p[1] != r)
-						bad++;
-				} else {
-					uint32_t words = size >> 2;
-					p[0] = pat;
-					for (uint32_t k = 0; k < hold; k++)
-						p[1 + k % (words - 1)] = pat + k + 1;
-					if (p[0] != pat)
-						bad++;
-				}
-				ops++;
-			}
-		}
-		kma_wg_free<KMA_BATCH_BLOCK>(H, (void *)p, L, ctx);
-	}
-	wave_add_stats(stats, ops, nulls, bad, mallocs);
+	wl_batched(H, L, round0, nrounds, mode, fixed, seed, nwi, variant, stats);
 }
 
 extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed, uint64_t seed,
-			    uint64_t nwi, unsigned long long *d_stats, void *stream)
+			    uint64_t nwi, uint32_t variant, unsigned long long *d_stats, void *stream)
 {
 	uint64_t blocks = (nwi + KMA_BATCH_BLOCK - 1) / KMA_BATCH_BLOCK;
 	hipLaunchKernelGGL(k_batched, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, (hipStream_t)stream,
-			   (kma_dheap *)heap, round0, nrounds, mode, fixed, seed, nwi, d_stats);
+			   (kma_dheap *)heap, round0, nrounds, mode, fixed, seed, nwi, variant, d_stats);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }

@@ -29,7 +29,7 @@ int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds,
 		uint32_t block, unsigned long long *d_stats, void *stream);
 
 int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode,
-		 uint32_t fixed, uint64_t seed, uint64_t nwi,
+		 uint32_t fixed, uint64_t seed, uint64_t nwi, uint32_t variant,
 		 unsigned long long *d_stats, void *stream);
 
 int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses);

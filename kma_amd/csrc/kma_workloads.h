@@ -1,0 +1,200 @@
+/*
+ * kma_workloads.h - device bodies of the per-work-item workloads (plain and
+ * fused forms, per-lane API) and of the batched form (work-group collective
+ * API).  Kept apart from the kernels so the same code also builds for the
+ * CPU simulation used by the tests.
+ */
+#ifndef KMA_WORKLOADS_H
+#define KMA_WORKLOADS_H
+
+#include "kma_device.h"
+#include "kma_ref_device.h"
+#include "kma_wg_device.h"
+#include "kma_work.h"
+
+/* ------------------------------------------------------------------ */
+/* deterministic trace values (same function as the host side)         */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ uint64_t kma_mix(uint64_t seed, uint64_t gid, uint64_t round, uint64_t k)
+{
+	uint64_t z = seed * 0x9E3779B97F4A7C15ull + gid * 0xBF58476D1CE4E5B9ull +
+		     round * 0x94D049BB133111EBull + k * 0xD6E8FEB86659FD93ull;
+	z += 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void wave_add_stats(unsigned long long *stats, uint64_t a, uint64_t b, uint64_t c,
+					       uint64_t d)
+{
+	/* one atomic per counter per wavefront */
+	uint64_t v[4] = {a, b, c, d};
+	for (int k = 0; k < 4; k++) {
+		uint64_t x = v[k];
+		for (int off = 32; off > 0; off >>= 1)
+			x += __shfl_xor(x, off);
+		if (__lane_id() == 0 && x)
+			atomicAdd(stats + k, (unsigned long long)x);
+	}
+}
+
+/*
+ * One kernel body for every per-work-item workload:
+ *  KMA_WL_FIXED    : malloc(fixed), write {gid, round}, read back, free
+ *                    (plain small workload)
+ *  KMA_WL_LOGUNI   : malloc(2^e), e uniform in 3..11, hold 0..64 writes,
+ *                    read back, free (full-occupancy workload)
+ *  KMA_WL_REF_TEST : kma_test_malloc (kma.cl:424-461): malloc(8), write pid,
+ *                    free; a NULL ends the work-item
+ *  KMA_WL_LOWVAR   : kma_test_malloc_lowvar (kma.cl:463-492)
+ *  KMA_WL_HIGHVAR  : kma_test_malloc_highvar (kma.cl:494-526)
+ * stats: [0] heap ops issued, [1] NULL returns, [2] read-back mismatches,
+ *        [3] mallocs issued.
+ */
+template <class HeapT>
+__device__ inline void wl_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed,
+				 uint64_t seed, uint64_t nwi, unsigned long long *stats)
+{
+	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
+	bool alive = gid < nwi;
+	for (uint32_t r = round0; r < round0 + nrounds; r++) {
+		if (!alive)
+			break;
+		if (mode == KMA_WL_FIXED || mode == KMA_WL_LOGUNI) {
+			uint32_t size, hold = 0;
+			if (mode == KMA_WL_FIXED) {
+				size = fixed;
+			} else {
+				size = 1u << (3 + kma_mix(seed, gid, r, 0) % 9);
+				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
+			}
+			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, size);
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+				continue;
+			}
+			uint32_t pat = (uint32_t)(gid << 8) ^ r;
+			if (mode == KMA_WL_FIXED) {
+				p[0] = (uint32_t)gid;
+				p[1] = r;
+				if (p[0] != (uint32_t)gid || p[1] != r)
+					bad++;
+			} else {
+				uint32_t words = size >> 2;
+				p[0] = pat;
+				for (uint32_t k = 0; k < hold; k++)
+					p[1 + k % (words - 1)] = pat + k + 1;
+				if (p[0] != pat)
+					bad++;
+			}
+			kma_free(heap, (void *)p);
+			ops++;
+		} else if (mode == KMA_WL_REF_TEST) {
+			volatile uint64_t *p = (volatile uint64_t *)kma_malloc(heap, 8);
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+				alive = false;
+				continue;
+			}
+			p[0] = gid;
+			kma_free(heap, (void *)p);
+			ops++;
+		} else {
+			uint32_t amount;
+			if (mode == KMA_WL_LOWVAR)
+				amount = (gid & 1) ? 8 : 16;
+			else
+				amount = 4u << ((gid + r) % 5);
+			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, amount);
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+				alive = false;
+				continue;
+			}
+			p[0] = mode == KMA_WL_LOWVAR ? (uint32_t)gid : 4919u;
+			if (gid == 0) {
+				volatile uint32_t *q = (volatile uint32_t *)kma_malloc(heap, 4000);
+				mallocs++;
+				ops++;
+				if (!q) {
+					nulls++;
+					alive = false;
+					continue;
+				}
+				q[8] = 4919;
+				kma_free(heap, (void *)q);
+				ops++;
+			}
+			kma_free(heap, (void *)p);
+			ops++;
+		}
+	}
+	wave_add_stats(stats, ops, nulls, bad, mallocs);
+}
+
+#define KMA_BATCH_BLOCK 256
+
+__device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, uint32_t round0, uint32_t nrounds,
+				  int mode, uint32_t fixed, uint64_t seed, uint64_t nwi, uint32_t variant,
+				  unsigned long long *stats)
+{
+	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	bool active = gid < nwi;
+	uint32_t ctx = blockIdx.x & (H->nctx - 1u);
+	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
+	for (uint32_t r = round0; r < round0 + nrounds; r++) {
+		uint32_t size = 0, hold = 0;
+		if (active) {
+			if (mode == KMA_WL_FIXED) {
+				size = fixed;
+			} else {
+				size = 1u << (3 + kma_mix(seed, gid, r, 0) % 9);
+				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
+			}
+		}
+		volatile uint32_t *p;
+		if (variant & 1u)
+			p = (volatile uint32_t *)kma_malloc(H, size);
+		else
+			p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, ctx);
+		if (active) {
+			mallocs++;
+			ops++;
+			if (!p) {
+				nulls++;
+			} else {
+				uint32_t pat = (uint32_t)(gid << 8) ^ r;
+				if (mode == KMA_WL_FIXED) {
+					p[0] = (uint32_t)gid;
+					p[1] = r;
+					if (p[0] != (uint32_t)gid || p[1] != r)
+						bad++;
+				} else {
+					uint32_t words = size >> 2;
+					p[0] = pat;
+					for (uint32_t k = 0; k < hold; k++)
+						p[1 + k % (words - 1)] = pat + k + 1;
+					if (p[0] != pat)
+						bad++;
+				}
+				ops++;
+			}
+		}
+		if (variant & 2u)
+			kma_free(H, (void *)p);
+		else
+			kma_wg_free<KMA_BATCH_BLOCK>(H, (void *)p, L, ctx);
+	}
+	wave_add_stats(stats, ops, nulls, bad, mallocs);
+}
+
+
+#endif /* KMA_WORKLOADS_H */

@@ -1,0 +1,160 @@
+/*
+ * CPU simulation of the device allocator's work-group collective path.
+ *
+ * Builds the library's own device headers (kma_device.h, kma_wg_device.h,
+ * kma_workloads.h) against the CPU stand-in for hip_runtime.h in this
+ * directory and runs the batched workload body on a native heap that lives
+ * in host memory, one OS thread per device thread.  Used by the CPU tests to
+ * check the allocator's invariants without a GPU.
+ *
+ * usage: kma_sim sb_log2 sblocks nwg nwi rounds mode seed nctx npool
+ * prints: ops nulls bad double_frees bad_frees waits canonical_problems
+ */
+#include <stdio.h>
+#include <string.h>
+
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "kma_layout.h"
+#include "kma_workloads.h"
+
+thread_local sim_lane g_sim_lane;
+dim3 g_sim_block_dim, g_sim_grid_dim;
+
+static void nat_init(kma_dheap *H)
+{
+	using namespace kma_nat;
+	for (uint64_t i = 0; i < H->nsb; i++) {
+		uint64_t nx = i + H->npool;
+		*pnext(H, i) = nx < H->nsb ? (uint32_t)(nx + 1u) : 0u;
+	}
+	for (uint32_t i = 0; i < H->npool; i++) {
+		uint64_t *hp = pool(H, i);
+		hp[0] = i < H->nsb ? (uint64_t)(i + 1u) : 0u;
+		hp[1] = i < H->nsb ? (H->nsb - i + H->npool - 1u) / H->npool : 0u;
+	}
+}
+
+static void trim(kma_dheap *H)
+{
+	using namespace kma_nat;
+	for (uint32_t ctx = 0; ctx < H->nctx; ctx++)
+		for (uint32_t c = 0; c < H->nclasses; c++) {
+			uint64_t h = __hip_atomic_exchange(head(H, ctx, c), HEAD_EMPTY, 0, 0);
+			if (h <= HEAD_BUSY)
+				continue;
+			uint32_t sb = (uint32_t)h - 2u;
+			uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
+			if (st_count(s) == 0 && st_cls1(s) != 0)
+				try_release(H, sb, s, ctx);
+		}
+}
+
+static int canonical_problems(kma_dheap *H)
+{
+	using namespace kma_nat;
+	int probs = 0;
+	for (uint64_t sb = 0; sb < H->nsb; sb++) {
+		if (*state(H, sb) & ((1ull << 40) - 1)) {
+			if (probs < 5)
+				fprintf(stderr, "sb %llu state %#llx\n", (unsigned long long)sb,
+					(unsigned long long)*state(H, sb));
+			probs++;
+		}
+		for (uint32_t w = 0; w < H->bm_words; w++)
+			if (bitmap(H, sb)[w]) {
+				if (probs < 5)
+					fprintf(stderr, "sb %llu word %u = %#x\n", (unsigned long long)sb, w,
+						bitmap(H, sb)[w]);
+				probs++;
+			}
+	}
+	std::vector<uint8_t> seen(H->nsb, 0);
+	uint64_t n = 0;
+	for (uint32_t s = 0; s < H->npool; s++) {
+		uint32_t top = (uint32_t)*pool(H, s);
+		uint64_t walked = 0;
+		while (top && walked <= H->nsb) {
+			if (seen[top - 1]++)
+				probs++;
+			n++;
+			walked++;
+			top = *pnext(H, top - 1);
+		}
+		if (walked != pool(H, s)[1])
+			probs++;
+	}
+	if (n != H->nsb)
+		probs++;
+	return probs;
+}
+
+int main(int argc, char **argv)
+{
+	if (argc < 10) {
+		fprintf(stderr, "usage: %s sb_log2 sblocks nwg nwi rounds mode seed nctx npool\n", argv[0]);
+		return 2;
+	}
+	kma_config in;
+	memset(&in, 0, sizeof(in));
+	in.layout = KMA_LAYOUT_NATIVE;
+	in.sb_log2 = (uint32_t)atoi(argv[1]);
+	in.sblocks = strtoull(argv[2], 0, 0);
+	uint32_t nwg = (uint32_t)atoi(argv[3]);
+	uint64_t nwi = strtoull(argv[4], 0, 0);
+	uint32_t rounds = (uint32_t)atoi(argv[5]);
+	int mode = atoi(argv[6]);
+	uint64_t seed = strtoull(argv[7], 0, 0);
+	in.nctx = (uint32_t)atoi(argv[8]);
+	in.npool = (uint32_t)atoi(argv[9]);
+	kma_config cfg;
+	nat_hdr nh;
+	char err[200];
+	if (kma_native_layout(&in, &cfg, &nh, err, sizeof(err)) != 0) {
+		fprintf(stderr, "layout: %s\n", err);
+		return 2;
+	}
+	uint64_t total = kma_native_alloc_bytes(&nh);
+	char *buf = (char *)aligned_alloc(4096, (total + 4095) & ~4095ull);
+	memset(buf, 0, total);
+	kma_native_place(&nh, (uint64_t)(uintptr_t)buf, total);
+	memcpy(buf, &nh, sizeof(nh));
+	kma_dheap *H = (kma_dheap *)buf;
+	nat_init(H);
+
+	g_sim_block_dim = {KMA_BATCH_BLOCK, 1, 1};
+	g_sim_grid_dim = {nwg, 1, 1};
+	unsigned long long stats[4] = {0, 0, 0, 0};
+	std::vector<std::unique_ptr<sim_wg>> wgs;
+	std::vector<std::unique_ptr<sim_wave>> waves;
+	std::vector<std::unique_ptr<kma_wg_lds<KMA_BATCH_BLOCK>>> lds;
+	std::vector<std::thread> th;
+	for (uint32_t b = 0; b < nwg; b++) {
+		wgs.emplace_back(new sim_wg(KMA_BATCH_BLOCK));
+		lds.emplace_back(new kma_wg_lds<KMA_BATCH_BLOCK>());
+		for (uint32_t w = 0; w < KMA_BATCH_BLOCK / 64; w++)
+			waves.emplace_back(new sim_wave());
+	}
+	for (uint32_t b = 0; b < nwg; b++)
+		for (uint32_t t = 0; t < KMA_BATCH_BLOCK; t++)
+			th.emplace_back([=, &wgs, &waves, &lds, &stats]() {
+				g_sim_lane.tid = {t, 0, 0};
+				g_sim_lane.bid = {b, 0, 0};
+				g_sim_lane.wg = wgs[b].get();
+				g_sim_lane.wave = waves[b * (KMA_BATCH_BLOCK / 64) + t / 64].get();
+				wl_batched(H, *lds[b], 0, rounds, mode, 64, seed, nwi, 0, stats);
+			});
+	for (auto &t : th)
+		t.join();
+	uint64_t *st = kma_nat::stats(H);
+	trim(H);
+	int probs = canonical_problems(H);
+	printf("ops %llu nulls %llu bad %llu double_frees %llu bad_frees %llu waits %llu %llu %llu %llu canonical_problems %d\n",
+	       stats[0], stats[1], stats[2], (unsigned long long)st[0], (unsigned long long)st[1],
+	       (unsigned long long)st[3], (unsigned long long)st[4], (unsigned long long)st[5],
+	       (unsigned long long)st[6], probs);
+	free(buf);
+	return 0;
+}

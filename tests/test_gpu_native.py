@@ -123,3 +123,18 @@ def test_batched_rejects_reference_layout():
     with kma_amd.Heap(kma_amd.LAYOUT_REFERENCE, 12, 64) as h:
         with pytest.raises(kma_amd.KmaError):
             h.run(kma_amd.WL_FIXED, 64, 1, form=kma_amd.FORM_BATCHED, block=256)
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_batched_mixed_with_per_lane_api(variant):
+    """Blocks from the work-group collective malloc freed by the per-lane
+    kma_free (variant 2), and the reverse (variant 1): both forms share one
+    heap protocol."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 65536) as h:
+        _say("native batched variant=%d" % variant)
+        st = h.run(kma_amd.WL_LOGUNI, 65536, 3, form=kma_amd.FORM_BATCHED, block=256, seed=2,
+                   variant=variant)
+        assert st["ops"] == 2 * 3 * 65536 and st["nulls"] == 0 and st["bad"] == 0, st
+        assert st["double_frees"] == 0 and st["bad_frees"] == 0 and st["waits"] == [0, 0, 0, 0], st
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []

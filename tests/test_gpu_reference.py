@@ -115,9 +115,9 @@ def test_reference_bench_kernels(mode):
 
 def test_full_occupancy_workload_reference_layout():
     """Mixed classes on the reference layout, concurrently (1 GiB heap at
-    16 KiB superblocks, 65,536 work-items, 2 rounds)."""
+    16 KiB superblocks, 16,384 work-items, 2 rounds)."""
     with kma_amd.Heap(kma_amd.LAYOUT_REFERENCE, 14, 65536) as h:
         _say("ref loguniform concurrent")
-        st = h.run(kma_amd.WL_LOGUNI, 65536, 2, block=256, seed=1)
+        st = h.run(kma_amd.WL_LOGUNI, 16384, 2, block=256, seed=1)
         assert st["bad"] == 0 and st["nulls"] == 0
         assert refimg.ref_canonical_free(h.snapshot(), 14, 65536) == []
