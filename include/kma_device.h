@@ -164,6 +164,14 @@ __device__ __forceinline__ T f_and(T *p, T v)
 	return __hip_atomic_fetch_and(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+/* a - b if a > b, else 0.  Written without a compare-and-select: with a
+ * select, `a >= b ? 0 : min(n, a - b)` was folded into a plain wrapping
+ * subtraction inside large kernels by this toolchain's optimizer. */
+__device__ __forceinline__ uint32_t sat_sub(uint32_t a, uint32_t b)
+{
+	return a - min(a, b);
+}
+
 /* ---- wavefront helpers ---- */
 __device__ __forceinline__ uint32_t lane() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane()) - 1ull; }
@@ -325,7 +333,7 @@ __device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uin
 			uint64_t add = ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ST_OWNED | n;
 			uint64_t old = fadd(state(H, sb), add);
 			uint32_t cnt = st_count(old);
-			uint32_t got = cnt >= cap ? 0u : min(n, cap - cnt);
+			uint32_t got = min(n, sat_sub(cap, cnt));
 			if (got < n)
 				fsub(state(H, sb), (uint64_t)(n - got));
 			st_rel(hp, ((uint64_t)st_gen(old) << 32) | (uint64_t)(sb + 2u));
@@ -347,7 +355,7 @@ __device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uin
 			continue;
 		}
 		uint32_t cnt = st_count(old);
-		uint32_t got = cnt >= cap ? 0u : min(n, cap - cnt);
+		uint32_t got = min(n, sat_sub(cap, cnt));
 		if (got < n) {
 			undo(H, sb, n - got, ctx);
 			if (cas(hp, h, HEAD_EMPTY) == h) {

@@ -89,7 +89,7 @@ __device__ inline uint32_t wave_claim(const kma_dheap *H, uint32_t sb, uint32_t 
 			uint32_t tot;
 			uint32_t pre = wave_excl_scan(__popc(freeb), &tot);
 			uint32_t need = got - done;
-			uint32_t take = pre >= need ? 0u : min((uint32_t)__popc(freeb), need - pre);
+			uint32_t take = min((uint32_t)__popc(freeb), sat_sub(need, pre));
 			uint32_t mask = low_bits(freeb, take);
 			uint32_t won = 0;
 			if (mask)
@@ -121,7 +121,8 @@ __device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
  * size 0 means "no request" for that thread.  `ctx` is the allocation
  * context the work-group reserves from. */
 template <int BLOCK>
-__device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t ctx)
+__device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t ctx,
+				      uint32_t per_lane_claim = 0)
 {
 	using namespace kma_nat;
 	const uint32_t tid = threadIdx.x, nth = blockDim.x, nw = (nth + 63u) >> 6, w = tid >> 6;
@@ -164,7 +165,21 @@ __device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOC
 			if (!ok)
 				break;
 			uint32_t sb = rdlane(r.sb, 0), rb = rdlane(r.base, 0), got = rdlane(r.got, 0);
-			uint32_t done = wave_claim(H, sb, k, rb, got, ord + r0, L.res);
+			uint32_t done;
+			if (per_lane_claim) {
+				/* one claim per block: lane j takes ranks j, j+64, ... */
+				uint32_t mine = 0;
+				for (uint32_t q = lane(); q < got; q += 64) {
+					char *p = claim(H, sb, k, rb + q);
+					L.res[ord[r0 + q]] = p ? (uint64_t)(p - base(H)) : 0;
+					mine += p ? 1u : 0u;
+				}
+				uint32_t tot;
+				wave_excl_scan(mine, &tot);
+				done = tot;
+			} else {
+				done = wave_claim(H, sb, k, rb, got, ord + r0, L.res);
+			}
 			r0 += done;
 			if (done < got) {
 				/* give back what could not be claimed */

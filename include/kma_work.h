@@ -74,6 +74,15 @@ int kma_trace_serial(kma_heap *h, const int64_t *ops, uint64_t n, uint64_t *out,
 int kma_atomic_peak(uint64_t bytes, uint64_t nthreads, uint32_t iters, int pattern,
 		    uint32_t warmup, uint32_t reps, double *ops_per_s, double *ms);
 
+/* Self-check of the wavefront bitmap claim on a native heap (test hook):
+ * case i claims got[i] blocks of class cls[i] from superblock i, whose
+ * bitmap (bm_words words) is first set to bm_in[i]; returns the block
+ * offsets in res_out[i*64 ..], the final bitmap in bm_out and the number
+ * claimed in done_out[i].  Destroys the heap's contents. */
+int kma_check_wave_claim(kma_heap *h, uint32_t ncases, const uint32_t *cls, const uint32_t *base,
+			 const uint32_t *got, const uint32_t *bm_in, uint64_t *res_out,
+			 uint32_t *bm_out, uint32_t *done_out);
+
 /* Native layout: detach every context's superblock so that fully free
  * superblocks return to the pool (no allocation may be in flight). */
 int kma_trim(kma_heap *h);

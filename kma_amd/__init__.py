@@ -25,7 +25,7 @@ EXPORTS = [
     "kma_create", "kma_create_ex", "kma_destroy", "kma_device_heap",
     "kma_heap_bytes", "kma_get_config", "kma_snapshot", "kma_reset",
     "kma_last_error", "kma_device_ok", "kma_run_workload", "kma_trace_serial",
-    "kma_trim",
+    "kma_trim", "kma_atomic_peak", "kma_check_wave_claim",
 ]
 
 
@@ -114,6 +114,9 @@ def lib():
         L.kma_run_workload.argtypes = [vp, P(Workload), P(RunStats)]
         L.kma_trace_serial.argtypes = [vp, vp, u64, vp, vp]
         L.kma_trim.argtypes = [vp]
+        L.kma_check_wave_claim.argtypes = [vp, u32, vp, vp, vp, vp, vp, vp, vp]
+        L.kma_atomic_peak.argtypes = [u64, u64, u32, ctypes.c_int, u32, u32,
+                                      P(ctypes.c_double), P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -216,3 +219,27 @@ class Heap:
         st = RunStats()
         _check(lib().kma_run_workload(self._h, ctypes.byref(w), ctypes.byref(st)), "kma_run_workload")
         return st.as_dict()
+
+
+def check_wave_claim(heap, cls, base, got, bm_in):
+    """Run the wavefront bitmap claim on preset bitmaps (test hook)."""
+    n = len(cls)
+    cls = np.ascontiguousarray(cls, dtype=np.uint32)
+    base = np.ascontiguousarray(base, dtype=np.uint32)
+    got = np.ascontiguousarray(got, dtype=np.uint32)
+    bm_in = np.ascontiguousarray(bm_in, dtype=np.uint32)
+    res = np.zeros(n * 64, dtype=np.uint64)
+    bm_out = np.zeros_like(bm_in)
+    done = np.zeros(n, dtype=np.uint32)
+    _check(lib().kma_check_wave_claim(heap._h, n, cls.ctypes.data, base.ctypes.data, got.ctypes.data,
+                                      bm_in.ctypes.data, res.ctypes.data, bm_out.ctypes.data,
+                                      done.ctypes.data), "kma_check_wave_claim")
+    return res.reshape(n, 64), bm_out, done
+
+
+def atomic_peak(nbytes, nthreads, iters, pattern, warmup=2, reps=5):
+    ops = ctypes.c_double()
+    ms = ctypes.c_double()
+    _check(lib().kma_atomic_peak(nbytes, nthreads, iters, pattern, warmup, reps, ctypes.byref(ops),
+                                 ctypes.byref(ms)), "kma_atomic_peak")
+    return ops.value, ms.value

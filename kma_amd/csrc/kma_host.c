@@ -493,3 +493,51 @@ done:
 		hipFree(buf);
 	return rc;
 }
+
+int kma_check_wave_claim(kma_heap *h, uint32_t n, const uint32_t *cls, const uint32_t *base, const uint32_t *got,
+			 const uint32_t *bm_in, uint64_t *res_out, uint32_t *bm_out, uint32_t *done_out)
+{
+	if (!h || h->cfg.layout != KMA_LAYOUT_NATIVE || n == 0 || n > h->nh.nsb) {
+		set_err("bad arguments");
+		return -1;
+	}
+	uint32_t W = h->nh.bm_words;
+	for (uint32_t i = 0; i < n; i++) {
+		if (cls[i] >= h->nh.nclasses || got[i] > 64) {
+			set_err("case %u: class or count out of range", i);
+			return -1;
+		}
+	}
+	void *d[7] = {0};
+	size_t sz[7] = {n * 4ull, n * 4ull, n * 4ull, (size_t)n * W * 4, n * 64ull * 8, (size_t)n * W * 4, n * 4ull};
+	int rc = -1;
+	for (int k = 0; k < 7; k++)
+		if (hipMalloc(&d[k], sz[k]) != hipSuccess) {
+			set_err("hipMalloc failed");
+			goto out;
+		}
+	if (hipMemcpy(d[0], cls, sz[0], hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(d[1], base, sz[1], hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(d[2], got, sz[2], hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(d[3], bm_in, sz[3], hipMemcpyHostToDevice) != hipSuccess) {
+		set_err("upload failed");
+		goto out;
+	}
+	rc = kmad_check_wave_claim(h->dev, n, d[0], d[1], d[2], d[3], d[4], d[5], d[6]);
+	if (rc) {
+		set_err("check kernel failed: %d", rc);
+		rc = -1;
+		goto out;
+	}
+	if (hipMemcpy(res_out, d[4], sz[4], hipMemcpyDeviceToHost) != hipSuccess ||
+	    hipMemcpy(bm_out, d[5], sz[5], hipMemcpyDeviceToHost) != hipSuccess ||
+	    hipMemcpy(done_out, d[6], sz[6], hipMemcpyDeviceToHost) != hipSuccess) {
+		set_err("download failed");
+		rc = -1;
+	}
+out:
+	for (int k = 0; k < 7; k++)
+		if (d[k])
+			hipFree(d[k]);
+	return rc;
+}

@@ -260,3 +260,40 @@ extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int m
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* wave_claim check: one wavefront per case, superblock `case` of a     */
+/* native heap preloaded with a bitmap; results back to the host        */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, const uint32_t *cls, const uint32_t *base,
+							 const uint32_t *got, const uint32_t *bm_in,
+							 uint64_t *res_out, uint32_t *bm_out, uint32_t *done_out)
+{
+	__shared__ uint16_t order[64];
+	__shared__ uint64_t res[64];
+	uint32_t cs = blockIdx.x, l = threadIdx.x;
+	uint32_t *bm = kma_nat::bitmap(H, cs);
+	for (uint32_t w = l; w < H->bm_words; w += 64)
+		bm[w] = bm_in[cs * H->bm_words + w];
+	order[l] = (uint16_t)l;
+	res[l] = 0;
+	__syncthreads();
+	uint32_t done = kma_nat::wave_claim(H, cs, cls[cs], base[cs], got[cs], order, res);
+	__syncthreads();
+	res_out[cs * 64 + l] = res[l];
+	for (uint32_t w = l; w < H->bm_words; w += 64)
+		bm_out[cs * H->bm_words + w] = kma_nat::ld_rlx(bm + w);
+	if (l == 0)
+		done_out[cs] = done;
+}
+
+extern "C" int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t *cls, const uint32_t *base,
+				     const uint32_t *got, const uint32_t *bm_in, uint64_t *res_out, uint32_t *bm_out,
+				     uint32_t *done_out)
+{
+	hipLaunchKernelGGL(k_check_wave_claim, dim3(ncases), dim3(64), 0, 0, (kma_dheap *)heap, cls, base, got, bm_in,
+			   res_out, bm_out, done_out);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}

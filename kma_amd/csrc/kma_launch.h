@@ -34,6 +34,10 @@ int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode,
 
 int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses);
 
+int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t *cls, const uint32_t *base,
+			  const uint32_t *got, const uint32_t *bm_in, uint64_t *res_out,
+			  uint32_t *bm_out, uint32_t *done_out);
+
 int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, uint32_t iters,
 		     int pattern, void *stream);
 

@@ -164,7 +164,7 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 		if (variant & 1u)
 			p = (volatile uint32_t *)kma_malloc(H, size);
 		else
-			p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, ctx);
+			p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, ctx, (variant >> 2) & 1u);
 		if (active) {
 			mallocs++;
 			ops++;

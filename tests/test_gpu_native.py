@@ -102,8 +102,6 @@ def test_full_occupancy_native(form):
         assert natimg.NatHeap(h).canonical_problems() == []
 
 
-@pytest.mark.skip(reason="went silent on the GPU (no output for 180 s) with the first batched "
-                         "implementation; no cause shown yet, so the case stays off the GPU")
 def test_batched_plain_small_and_partial_workgroup():
     """Batched form with a work-item count that leaves the last work-group
     partly empty, and the fixed-size workload."""
@@ -136,5 +134,65 @@ def test_batched_mixed_with_per_lane_api(variant):
                    variant=variant)
         assert st["ops"] == 2 * 3 * 65536 and st["nulls"] == 0 and st["bad"] == 0, st
         assert st["double_frees"] == 0 and st["bad_frees"] == 0 and st["waits"] == [0, 0, 0, 0], st
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
+def _expected_claim(bm, cap, base, got, d0, size):
+    """Model of the wavefront claim with no concurrent claimer: words from
+    (base/32) mod W onward, lowest free bits first, in rank order."""
+    W = (cap + 31) // 32
+    w0 = (base >> 5) % W
+    out, bm = [], list(bm)
+    for k in range(W):
+        wi = (w0 + k) % W
+        valid = cap - wi * 32
+        for b in range(min(32, valid)):
+            if len(out) == got:
+                break
+            if not (bm[wi] >> b) & 1:
+                bm[wi] |= 1 << b
+                out.append(d0 + (wi * 32 + b) * size)
+    return out, bm
+
+
+def test_wave_claim_matches_model():
+    rng = np.random.default_rng(11)
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=1) as h:
+        nh = natimg.NatHeap(h)
+        W = nh.bm_words
+        cls, base, got, bms = [], [], [], []
+        for i in range(64):
+            c = i % nh.nclasses
+            cap = nh.caps[c]
+            words = (cap + 31) // 32
+            dens = rng.choice([0.0, 0.3, 0.7, 0.95])
+            bits = (rng.random(words * 32) < dens)
+            bits[cap:] = False
+            bm = np.zeros(W, dtype=np.uint32)
+            for wi in range(words):
+                bm[wi] = int(sum(1 << b for b in range(32) if bits[wi * 32 + b]))
+            nfree = cap - int(bits.sum())
+            cls.append(c)
+            base.append(int(rng.integers(0, cap)))
+            got.append(int(min(nfree, rng.integers(0, 65))))
+            bms.append(bm)
+        res, bm_out, done = kma_amd.check_wave_claim(h, cls, base, got, np.concatenate(bms))
+        for i in range(64):
+            c = cls[i]
+            d0 = nh.off_data + (i << nh.sb_log2)
+            want, wbm = _expected_claim([int(x) for x in bms[i]], nh.caps[c], base[i], got[i], d0,
+                                        nh.classes[c])
+            assert done[i] == got[i], (i, c, done[i], got[i])
+            assert [int(x) for x in res[i][:got[i]]] == want, (i, c, base[i], got[i])
+            assert [int(x) for x in bm_out[i * W:(i + 1) * W]] == [int(x) for x in wbm[:W]], i
+
+
+def test_batched_single_workgroup():
+    """One work-group, no other allocator activity: the collective malloc and
+    free on mixed classes."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 4096) as h:
+        st = h.run(kma_amd.WL_LOGUNI, 256, 20, form=kma_amd.FORM_BATCHED, block=256, seed=3)
+        assert st["ops"] == 2 * 20 * 256 and st["nulls"] == 0 and st["bad"] == 0, st
         h.trim()
         assert natimg.NatHeap(h).canonical_problems() == []
