@@ -7,7 +7,7 @@
  * uniformity requirement; NULL on failure; kma_free(heap, NULL) is a no-op.
  *
  * Heap layout (one device buffer, all offsets from the header):
- *   kma_dheap header | u64 state[nsb] | u32 pool_next[nsb] |
+ *   kma_dheap header | stats | u64 state[nsb] | u32 pool_next[nsb] |
  *   u32 bitmap[nsb][bm_words] | u64 heads[nctx][KMA_HEADS_PER_CTX] |
  *   pool shards (128 B each) | u64 pbitmap[npages/64] | u8 plen[npages] |
  *   superblock data (nsb << sb_log2, SB-aligned) | page data.
@@ -18,7 +18,8 @@
  *   bits  0..31  count: live blocks + in-flight reservations
  *   bits 32..36  class + 1 (0: superblock is free / in the pool)
  *   bit  37      owned: attached to a context head
- *   bits 40..63  generation, bumped each time the superblock returns to the pool
+ *   bits 38..40  home XCD: the XCD whose context attached it
+ *   bits 41..63  generation, bumped each time the superblock returns to the pool
  * A wavefront group of n same-class lanes reserves with ONE atomicAdd(n) on
  * this word; the returned old value says (atomically) whether the superblock
  * is still the incarnation the group read from the context head (generation),
@@ -28,11 +29,30 @@
  * generation.  Over-reservations are undone with atomicSub; a transiently
  * inflated count only makes a superblock look fuller than it is.
  *
+ * Memory ordering.  Every allocator atomic is a relaxed agent-scope atomic:
+ * RMWs are performed at the device-coherent level, and each protocol step
+ * that must follow another consumes the earlier step's returned value.  An
+ * agent-scope acquire/release on this part writes back / invalidates the
+ * whole L2 of the XCD, so it is used only where a block's cache lines can
+ * move to another XCD:
+ *   - contexts and superblocks are homed on one XCD (contexts are chosen by
+ *     the XCC id of the calling wavefront), so a block is normally reused on
+ *     the XCD whose L2 holds its lines;
+ *   - a free from another XCD first writes back its own L2 (release);
+ *   - the last free of a superblock on its home XCD writes back the L2 and
+ *     returns the superblock to a "clean" pool shard any XCD may take from;
+ *     if the last free came from another XCD, the superblock goes to its home
+ *     XCD's "dirty" shard, which only the home XCD pops from;
+ *   - page frees always write back (pages are not homed).
+ * A local free only waits for its own stores (s_waitcnt) before it clears
+ * the block's bit.
+ *
  * Context heads (64-bit): 0 empty, 1 refill in progress, otherwise
  * (gen << 32) | (sb + 2).  Pool shards are Treiber stacks with a 64-bit
- * {top, tag} head (ABA tag bumped on every pop and push).
- * Large requests (> largest class) take 1..64 contiguous pages from a page
- * bitmap (64-bit words, runs aligned to the next power of two).
+ * {tag, top} head (ABA tag bumped on every pop and push); XCD x owns shards
+ * [x*S, x*S + S) with S = npool/8: shard x*S is its dirty shard, the rest are
+ * clean.  Large requests (> largest class) take 1..64 contiguous pages from a
+ * page bitmap (64-bit words, runs aligned to the next power of two).
  */
 #ifndef KMA_DEVICE_H
 #define KMA_DEVICE_H
@@ -43,6 +63,7 @@
 #define KMA_NATIVE_MAGIC 0x4e414d4b31414d4bull /* "KMA1KMAN" */
 #define KMA_HEADS_PER_CTX 32u
 #define KMA_POOL_STRIDE 16u /* u64 words per pool shard (128 B) */
+#define KMA_NXCD 8u         /* XCDs per MI355X */
 
 typedef struct kma_dheap {
 	uint64_t magic;
@@ -81,6 +102,7 @@ typedef struct kma_dheap {
 #define KMA_STAT_WAIT_CLAIM 4   /* a per-lane bitmap claim hit the watchdog */
 #define KMA_STAT_WAIT_WGCLAIM 5 /* a wavefront bitmap claim hit the watchdog */
 #define KMA_STAT_WAIT_POOL 6    /* a pool push/pop hit the watchdog */
+#define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
 #define KMA_NSTATS 8
 
 /* Every wait in the native allocator gives up after this many ticks of the
@@ -97,7 +119,9 @@ static constexpr uint64_t ST_COUNT = 0xffffffffull;
 static constexpr uint64_t ST_CLASS_SHIFT = 32;
 static constexpr uint64_t ST_CLASS = 0x1full << ST_CLASS_SHIFT;
 static constexpr uint64_t ST_OWNED = 1ull << 37;
-static constexpr uint64_t ST_GEN_SHIFT = 40;
+static constexpr uint64_t ST_HOME_SHIFT = 38;
+static constexpr uint64_t ST_GEN_SHIFT = 41;
+static constexpr uint32_t ST_GEN_MASK = 0x7fffffu;
 
 /* Watchdog for retry loops: reads the wall clock every 256 retries only. */
 struct waiter {
@@ -118,50 +142,63 @@ struct waiter {
 
 __device__ __forceinline__ uint32_t st_count(uint64_t s) { return (uint32_t)(s & ST_COUNT); }
 __device__ __forceinline__ uint32_t st_cls1(uint64_t s) { return (uint32_t)((s & ST_CLASS) >> ST_CLASS_SHIFT); }
+__device__ __forceinline__ uint32_t st_home(uint64_t s) { return (uint32_t)(s >> ST_HOME_SHIFT) & 7u; }
 __device__ __forceinline__ uint32_t st_gen(uint64_t s) { return (uint32_t)(s >> ST_GEN_SHIFT); }
 
-/* ---- agent-scope atomics ---- */
+/* ---- relaxed agent-scope atomics ---- */
 template <typename T>
-__device__ __forceinline__ T ld_acq(const T *p)
-{
-	return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T ld_rlx(const T *p)
+__device__ __forceinline__ T ld(const T *p)
 {
 	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
-__device__ __forceinline__ void st_rel(T *p, T v)
+__device__ __forceinline__ void st(T *p, T v)
 {
-	__hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T cas(T *p, T cmp, T val)
 {
-	__hip_atomic_compare_exchange_strong(p, &cmp, val, __ATOMIC_ACQ_REL,
-					     __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+	__hip_atomic_compare_exchange_strong(p, &cmp, val, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+					     __HIP_MEMORY_SCOPE_AGENT);
 	return cmp;
+}
+template <typename T>
+__device__ __forceinline__ T xchg(T *p, T v)
+{
+	return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T fadd(T *p, T v)
 {
-	return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+	return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T fsub(T *p, T v)
 {
-	return __hip_atomic_fetch_sub(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+	return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T f_or(T *p, T v)
 {
-	return __hip_atomic_fetch_or(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+	return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T f_and(T *p, T v)
 {
-	return __hip_atomic_fetch_and(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+	return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Wait until this thread's earlier memory operations have completed
+ * (workgroup-scope release: s_waitcnt only, no cache maintenance). */
+__device__ __forceinline__ void order_prior()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+}
+/* Agent-scope release: also writes back this XCD's L2. */
+__device__ __forceinline__ void release_l2()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 /* a - b if a > b, else 0.  Written without a compare-and-select: with a
@@ -172,16 +209,18 @@ __device__ __forceinline__ uint32_t sat_sub(uint32_t a, uint32_t b)
 	return a - min(a, b);
 }
 
+/* XCD (XCC) the calling wavefront runs on: hwreg HW_REG_XCC_ID bits 0..3. */
+__device__ __forceinline__ uint32_t xcc()
+{
+	return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & (KMA_NXCD - 1u);
+}
+
 /* ---- wavefront helpers ---- */
 __device__ __forceinline__ uint32_t lane() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane()) - 1ull; }
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l)
 {
 	return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
-}
-__device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l)
-{
-	return ((uint64_t)rdlane((uint32_t)(v >> 32), l) << 32) | rdlane((uint32_t)v, l);
 }
 
 /* ---- layout accessors ---- */
@@ -212,31 +251,53 @@ __device__ __forceinline__ int class_of(const kma_dheap *H, uint32_t size)
 	return c;
 }
 
-/* Allocation context of the calling wavefront: contiguous waves share one,
- * so the per-class superblock a context holds is hit by few waves. */
-__device__ __forceinline__ uint32_t ctx_of(const kma_dheap *H)
+/* Contexts of XCD x are [x*P, x*P + P), P = nctx/8.  Within its XCD, a
+ * wavefront maps to a context by (work-group, wave) so that a few nearby
+ * wavefronts share each context's per-class superblocks. */
+__device__ __forceinline__ uint32_t ctx_of(const kma_dheap *H, uint32_t x)
 {
+	uint32_t per = H->nctx / KMA_NXCD;
 	uint32_t wpb = (blockDim.x * blockDim.y * blockDim.z + 63u) >> 6;
 	uint32_t tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 	uint32_t bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-	return (bid * wpb + (tid >> 6)) & (H->nctx - 1u);
+	return x * per + ((bid / KMA_NXCD) * wpb + (tid >> 6)) % per;
+}
+
+/* Context of a whole work-group (work-group collective API). */
+__device__ __forceinline__ uint32_t wg_ctx_of(const kma_dheap *H, uint32_t x)
+{
+	uint32_t per = H->nctx / KMA_NXCD;
+	uint32_t bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+	return x * per + (bid / KMA_NXCD) % per;
 }
 
 /* ---- superblock pool: sharded Treiber stacks, 64-bit {tag:32, top:32} ---- */
+__device__ __forceinline__ uint32_t shards_per_xcd(const kma_dheap *H) { return H->npool / KMA_NXCD; }
+__device__ __forceinline__ uint32_t dirty_shard(const kma_dheap *H, uint32_t x) { return x * shards_per_xcd(H); }
+__device__ __forceinline__ uint32_t clean_shard(const kma_dheap *H, uint32_t x, uint32_t ctx)
+{
+	uint32_t S = shards_per_xcd(H);
+	return x * S + 1u + ctx % (S - 1u);
+}
+
 __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb)
 {
 	uint64_t *hp = pool(H, shard);
+	uint64_t h = ld(hp);
 	waiter wt;
 	for (;;) {
 		if (wt.expired()) {
 			bump_stat(H, KMA_STAT_WAIT_POOL);
 			return;
 		}
-		uint64_t h = ld_acq(hp);
-		st_rel(pnext(H, sb), (uint32_t)h);
-		uint64_t nh = ((h >> 32) + 1ull) << 32 | (uint64_t)(sb + 1u);
-		if (cas(hp, h, nh) == h)
+		/* the link must be performed before the head CAS publishes it */
+		xchg(pnext(H, sb), (uint32_t)h);
+		order_prior();
+		uint64_t nh = (((h >> 32) + 1ull) << 32) | (uint64_t)(sb + 1u);
+		uint64_t seen = cas(hp, h, nh);
+		if (seen == h)
 			break;
+		h = seen;
 	}
 	fadd(hp + 1, (uint64_t)1);
 }
@@ -244,60 +305,82 @@ __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb
 __device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32_t *out)
 {
 	uint64_t *hp = pool(H, shard);
+	uint64_t h = ld(hp);
 	waiter wt;
 	for (;;) {
 		if (wt.expired()) {
 			bump_stat(H, KMA_STAT_WAIT_POOL);
 			return false;
 		}
-		uint64_t h = ld_acq(hp);
 		uint32_t top = (uint32_t)h;
-		if (top == 0)
-			return false;
-		uint32_t nx = ld_acq(pnext(H, top - 1u));
-		uint64_t nh = ((h >> 32) + 1ull) << 32 | (uint64_t)nx;
-		if (cas(hp, h, nh) == h) {
+		if (top == 0) {
+			/* confirm emptiness with an RMW read of the head */
+			h = fadd(hp, (uint64_t)0);
+			if ((uint32_t)h == 0)
+				return false;
+			continue;
+		}
+		uint32_t nx = fadd(pnext(H, top - 1u), 0u);
+		uint64_t nh = (((h >> 32) + 1ull) << 32) | (uint64_t)nx;
+		uint64_t seen = cas(hp, h, nh);
+		if (seen == h) {
 			fsub(hp + 1, (uint64_t)1);
 			*out = top - 1u;
 			return true;
 		}
+		h = seen;
 	}
 }
 
-__device__ inline bool pool_pop(const kma_dheap *H, uint32_t ctx, uint32_t *out)
+/* Pop for XCD x: its own clean shards, then its dirty shard, then the other
+ * XCDs' clean shards. */
+__device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t *out)
 {
-	uint32_t m = H->npool - 1u, s0 = ctx & m;
-	for (uint32_t i = 0; i < H->npool; i++) {
-		uint32_t s = (s0 + i) & m;
-		if ((uint32_t)ld_rlx(pool(H, s)) == 0)
-			continue;
-		if (pool_pop_shard(H, s, out))
+	uint32_t S = shards_per_xcd(H);
+	uint32_t c0 = clean_shard(H, x, ctx) - x * S - 1u;
+	for (uint32_t i = 0; i < S - 1u; i++)
+		if (pool_pop_shard(H, x * S + 1u + (c0 + i) % (S - 1u), out))
 			return true;
+	if (pool_pop_shard(H, dirty_shard(H, x), out))
+		return true;
+	for (uint32_t j = 1; j < KMA_NXCD; j++) {
+		uint32_t y = (x + j) & (KMA_NXCD - 1u);
+		for (uint32_t i = 0; i < S - 1u; i++)
+			if (pool_pop_shard(H, y * S + 1u + (c0 + i) % (S - 1u), out))
+				return true;
 	}
 	return false;
 }
 
-/* Called by whoever produced state `s` == {count 0, not owned, class set}. */
-__device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t ctx)
+/* Called by whoever produced state `s` == {count 0, not owned, class set},
+ * running on XCD x. */
+__device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t x, uint32_t ctx)
 {
 	uint32_t g = st_gen(s);
 	for (;;) {
-		uint64_t fresh = (uint64_t)((g + 1u) & 0xffffffu) << ST_GEN_SHIFT;
-		if (cas(state(H, sb), s, fresh) == s) {
-			pool_push(H, ctx & (H->npool - 1u), sb);
+		uint64_t fresh = (uint64_t)((g + 1u) & ST_GEN_MASK) << ST_GEN_SHIFT;
+		uint64_t seen = cas(state(H, sb), s, fresh);
+		if (seen == s) {
+			uint32_t home = st_home(s);
+			if (home == x) {
+				release_l2();
+				pool_push(H, clean_shard(H, x, ctx), sb);
+			} else {
+				pool_push(H, dirty_shard(H, home), sb);
+			}
 			return;
 		}
-		s = ld_acq(state(H, sb));
+		s = seen;
 		if (st_count(s) != 0 || (s & ST_OWNED) || st_cls1(s) == 0 || st_gen(s) != g)
 			return;
 	}
 }
 
-__device__ __forceinline__ void undo(const kma_dheap *H, uint32_t sb, uint32_t m, uint32_t ctx)
+__device__ __forceinline__ void undo(const kma_dheap *H, uint32_t sb, uint32_t m, uint32_t x, uint32_t ctx)
 {
 	uint64_t s = fsub(state(H, sb), (uint64_t)m) - m;
 	if (st_count(s) == 0 && !(s & ST_OWNED) && st_cls1(s) != 0)
-		try_release(H, sb, s, ctx);
+		try_release(H, sb, s, x, ctx);
 }
 
 struct resv {
@@ -308,35 +391,40 @@ struct resv {
 };
 
 /* Reserve up to n slots of class c from context ctx's current superblock
- * (one lane executes this for its wavefront group). */
-__device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uint32_t n)
+ * (one lane executes this for its wavefront or work-group group). */
+__device__ inline resv reserve(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c, uint32_t n)
 {
 	uint64_t *hp = head(H, ctx, c);
 	uint32_t cap = H->class_cap[c];
 	waiter wt;
 	uint32_t bo = 1;
+	uint64_t h = ld(hp);
 	for (;;) {
 		if (wt.expired()) {
 			bump_stat(H, KMA_STAT_WAIT_RESERVE);
 			return resv{0, 0, 0, 0};
 		}
-		uint64_t h = ld_acq(hp);
 		if (h == HEAD_EMPTY) {
-			if (cas(hp, HEAD_EMPTY, HEAD_BUSY) != HEAD_EMPTY)
+			uint64_t seen = cas(hp, HEAD_EMPTY, HEAD_BUSY);
+			if (seen != HEAD_EMPTY) {
+				h = seen;
 				continue;
+			}
 			uint32_t sb;
-			if (!pool_pop(H, ctx, &sb)) {
-				st_rel(hp, HEAD_EMPTY);
+			if (!pool_pop(H, x, ctx, &sb)) {
+				st(hp, HEAD_EMPTY);
 				bump_stat(H, KMA_STAT_POOL_EMPTY);
 				return resv{0, 0, 0, 0};
 			}
-			uint64_t add = ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ST_OWNED | n;
+			uint64_t add = ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ST_OWNED |
+				       ((uint64_t)x << ST_HOME_SHIFT) | n;
 			uint64_t old = fadd(state(H, sb), add);
 			uint32_t cnt = st_count(old);
 			uint32_t got = min(n, sat_sub(cap, cnt));
 			if (got < n)
 				fsub(state(H, sb), (uint64_t)(n - got));
-			st_rel(hp, ((uint64_t)st_gen(old) << 32) | (uint64_t)(sb + 2u));
+			h = ((uint64_t)st_gen(old) << 32) | (uint64_t)(sb + 2u);
+			st(hp, h);
 			if (got == 0)
 				continue;
 			return resv{sb, cnt, got, 1};
@@ -345,32 +433,40 @@ __device__ inline resv reserve(const kma_dheap *H, uint32_t ctx, uint32_t c, uin
 			for (uint32_t i = 0; i < bo; i++)
 				__builtin_amdgcn_s_sleep(4);
 			bo = bo < 64u ? 2u * bo : 64u;
+			h = fadd(hp, (uint64_t)0);
 			continue;
 		}
 		uint32_t sb = (uint32_t)h - 2u;
 		uint32_t g = (uint32_t)(h >> 32);
 		uint64_t old = fadd(state(H, sb), (uint64_t)n);
 		if (st_gen(old) != g) {
-			undo(H, sb, n, ctx);
+			undo(H, sb, n, x, ctx);
+			h = fadd(hp, (uint64_t)0);
 			continue;
 		}
 		uint32_t cnt = st_count(old);
 		uint32_t got = min(n, sat_sub(cap, cnt));
 		if (got < n) {
-			undo(H, sb, n - got, ctx);
-			if (cas(hp, h, HEAD_EMPTY) == h) {
+			undo(H, sb, n - got, x, ctx);
+			uint64_t seen = cas(hp, h, HEAD_EMPTY);
+			if (seen == h) {
 				uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
 				if (st_count(s) == 0 && st_cls1(s) != 0)
-					try_release(H, sb, s, ctx);
+					try_release(H, sb, s, x, ctx);
+				seen = HEAD_EMPTY;
 			}
-			if (got == 0)
+			if (got == 0) {
+				h = seen;
 				continue;
+			}
 		}
 		return resv{sb, cnt, got, 1};
 	}
 }
 
-/* Claim one free bit at or after `start` (wrapping) in superblock sb. */
+/* Claim one free bit at or after `start` (wrapping) in superblock sb.  The
+ * first pass reads the bitmap with plain atomic loads; later passes read it
+ * with RMWs, so a stale cached word cannot hide free bits for long. */
 __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t start)
 {
 	uint32_t cap = H->class_cap[c];
@@ -378,10 +474,11 @@ __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint3
 	uint32_t b = start;
 	uint32_t lastw = (cap - 1u) >> 5;
 	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
+	bool fresh = false;
 	waiter wt;
 	while (!wt.expired()) {
 		uint32_t wi = b >> 5;
-		uint32_t w = ld_rlx(bm + wi);
+		uint32_t w = fresh ? f_or(bm + wi, 0u) : ld(bm + wi);
 		uint32_t avail = ~w & (~0u << (b & 31u));
 		if (wi == lastw)
 			avail &= tail;
@@ -394,8 +491,10 @@ __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint3
 			continue;
 		}
 		b = (wi + 1u) << 5;
-		if (b >= cap)
+		if (b >= cap) {
 			b = 0;
+			fresh = true;
+		}
 	}
 	bump_stat(H, KMA_STAT_WAIT_CLAIM);
 	return nullptr;
@@ -426,8 +525,9 @@ __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t s
 	uint64_t nwords = (H->npages + 63) >> 6;
 	uint64_t *pb = pbits(H);
 	uint64_t wi = ((uint64_t)seed * 0x9E3779B1u) % nwords;
-	for (uint64_t probe = 0; probe < nwords;) {
-		uint64_t w = ld_rlx(pb + wi);
+	bool fresh = false;
+	for (uint64_t probe = 0; probe < 2 * nwords;) {
+		uint64_t w = fresh ? f_or(pb + wi, (uint64_t)0) : ld(pb + wi);
 		uint64_t cand = aligned_free_runs(w, kp);
 		if (wi == nwords - 1 && (H->npages & 63)) {
 			/* runs must end inside the page range */
@@ -439,7 +539,8 @@ __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t s
 		}
 		if (!cand) {
 			wi = (wi + 1 == nwords) ? 0 : wi + 1;
-			probe++;
+			if (++probe == nwords)
+				fresh = true; /* second sweep with RMW reads */
 			continue;
 		}
 		uint32_t j = __ffsll((unsigned long long)cand) - 1u;
@@ -447,7 +548,7 @@ __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t s
 		uint64_t old = f_or(pb + wi, m);
 		if ((old & m) == 0) {
 			uint64_t pi = (wi << 6) + j;
-			__hip_atomic_store(plen(H) + pi, (uint8_t)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			st(plen(H) + pi, (uint8_t)k);
 			return base(H) + H->off_pages + (pi << H->page_log2);
 		}
 		if (m & ~old)
@@ -459,12 +560,14 @@ __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t s
 __device__ inline void page_free(const kma_dheap *H, char *p)
 {
 	uint64_t pi = (uint64_t)(p - (base(H) + H->off_pages)) >> H->page_log2;
-	uint32_t k = __hip_atomic_load(plen(H) + pi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	uint32_t k = ld(plen(H) + pi);
 	if (k == 0 || k > 64) {
 		bump_stat(H, KMA_STAT_BAD_FREE);
 		return;
 	}
 	uint64_t m = ((k == 64) ? ~0ull : ((1ull << k) - 1ull)) << (pi & 63);
+	/* pages are reused by any XCD: write back this XCD's L2 first */
+	release_l2();
 	uint64_t old = f_and(pbits(H) + (pi >> 6), ~m);
 	if ((old & m) != m)
 		bump_stat(H, KMA_STAT_DOUBLE_FREE);
@@ -483,7 +586,8 @@ __device__ inline void *kma_malloc(kma_dheap *H, size_t size)
 		return page_alloc(H, size, seed);
 	}
 	uint32_t c = (uint32_t)class_of(H, (uint32_t)size);
-	uint32_t ctx = ctx_of(H);
+	uint32_t x = xcc();
+	uint32_t ctx = ctx_of(H, x);
 	char *result = nullptr;
 	bool pending = true;
 	for (;;) {
@@ -497,7 +601,7 @@ __device__ inline void *kma_malloc(kma_dheap *H, size_t size)
 		uint32_t n = (uint32_t)__popcll(grp);
 		resv r = {0, 0, 0, 0};
 		if (lane() == leader)
-			r = reserve(H, ctx, lc, n);
+			r = reserve(H, x, ctx, lc, n);
 		uint32_t sb = rdlane(r.sb, leader);
 		uint32_t rbase = rdlane(r.base, leader);
 		uint32_t got = rdlane(r.got, leader);
@@ -532,14 +636,21 @@ __device__ inline void kma_free(kma_dheap *H, void *ptr)
 			bump_stat(H, KMA_STAT_BAD_FREE);
 		return;
 	}
+	uint32_t x = xcc();
 	uint32_t sb = (uint32_t)(off >> H->sb_log2);
-	uint64_t s = ld_rlx(state(H, sb));
+	uint64_t s = ld(state(H, sb));
 	uint32_t c1 = st_cls1(s);
 	bool live = c1 != 0;
 	if (live) {
 		uint64_t inoff = off & ((1ull << H->sb_log2) - 1ull);
 		uint32_t idx = (uint32_t)((inoff * H->class_magic[c1 - 1u]) >> 40);
 		uint32_t m = 1u << (idx & 31u);
+		if (st_home(s) != x) {
+			bump_stat(H, KMA_STAT_REMOTE_FREE);
+			release_l2();
+		} else {
+			order_prior();
+		}
 		uint32_t old = f_and(bitmap(H, sb) + (idx >> 5), ~m);
 		if (!(old & m)) {
 			bump_stat(H, KMA_STAT_DOUBLE_FREE);
@@ -549,7 +660,7 @@ __device__ inline void kma_free(kma_dheap *H, void *ptr)
 		bump_stat(H, KMA_STAT_BAD_FREE);
 	}
 	/* one count update per superblock per wavefront */
-	uint32_t ctx = ctx_of(H);
+	uint32_t ctx = ctx_of(H, x);
 	bool pending = live;
 	for (;;) {
 		uint64_t act = __ballot(pending);
@@ -563,7 +674,7 @@ __device__ inline void kma_free(kma_dheap *H, void *ptr)
 			uint32_t n = (uint32_t)__popcll(grp);
 			uint64_t ns = fsub(state(H, lsb), (uint64_t)n) - n;
 			if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
-				try_release(H, lsb, ns, ctx);
+				try_release(H, lsb, ns, x, ctx);
 		}
 		if (member)
 			pending = false;

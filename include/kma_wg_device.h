@@ -74,15 +74,17 @@ __device__ inline uint32_t wave_claim(const kma_dheap *H, uint32_t sb, uint32_t 
 	uint32_t bsize = H->class_size[c];
 	uint32_t done = 0;
 	uint32_t w0 = (base >> 5) % W;
+	bool fresh = false; /* after a pass without progress, re-read with RMWs */
 	waiter wt;
 	while (done < got && !wt.expired()) {
+		uint32_t before = done;
 		for (uint32_t chunk = 0; chunk < W && done < got; chunk += 64) {
 			uint32_t k = chunk + lane();
 			bool has = k < W;
 			uint32_t wi = has ? (w0 + k) % W : 0;
 			uint32_t freeb = 0;
 			if (has) {
-				freeb = ~ld_rlx(bm + wi);
+				freeb = ~(fresh ? f_or(bm + wi, 0u) : ld(bm + wi));
 				if (wi == W - 1)
 					freeb &= tail;
 			}
@@ -104,6 +106,7 @@ __device__ inline uint32_t wave_claim(const kma_dheap *H, uint32_t sb, uint32_t 
 			}
 			done += wtot;
 		}
+		fresh = done == before;
 	}
 	if (done < got && lane() == 0)
 		bump_stat(H, KMA_STAT_WAIT_WGCLAIM);
@@ -118,14 +121,14 @@ __device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
 } /* namespace kma_nat */
 
 /* Work-group collective malloc.  All threads of the 1-D work-group call it;
- * size 0 means "no request" for that thread.  `ctx` is the allocation
- * context the work-group reserves from. */
+ * size 0 means "no request" for that thread.  per_lane_claim selects one
+ * bitmap atomic per block instead of one per bitmap word. */
 template <int BLOCK>
-__device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t ctx,
-				      uint32_t per_lane_claim = 0)
+__device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t per_lane_claim = 0)
 {
 	using namespace kma_nat;
 	const uint32_t tid = threadIdx.x, nth = blockDim.x, nw = (nth + 63u) >> 6, w = tid >> 6;
+	const uint32_t x = xcc(), ctx = wg_ctx_of(H, x);
 	const uint32_t ncls = H->nclasses;
 	if (tid < 32)
 		L.cnt[tid] = 0;
@@ -160,7 +163,7 @@ __device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOC
 		while (r0 < n) {
 			resv r = {0, 0, 0, 0};
 			if (lane() == 0)
-				r = reserve(H, ctx, k, n - r0);
+				r = reserve(H, x, ctx, k, n - r0);
 			uint32_t ok = rdlane(r.ok, 0);
 			if (!ok)
 				break;
@@ -184,7 +187,7 @@ __device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOC
 			if (done < got) {
 				/* give back what could not be claimed */
 				if (lane() == 0)
-					undo(H, sb, got - done, ctx);
+					undo(H, sb, got - done, x, ctx);
 				break;
 			}
 		}
@@ -206,10 +209,11 @@ __device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOC
 /* Work-group collective free.  All threads call it; NULL means "no
  * request".  Double frees are detected and counted, not applied. */
 template <int BLOCK>
-__device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L, uint32_t ctx)
+__device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L)
 {
 	using namespace kma_nat;
 	const uint32_t tid = threadIdx.x, nth = blockDim.x;
+	const uint32_t x = xcc(), ctx = wg_ctx_of(H, x);
 	const uint32_t HN = 2u * BLOCK;
 	for (uint32_t i = tid; i < HN; i += nth) {
 		L.hkey[i] = 0;
@@ -225,7 +229,7 @@ __device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L
 		uint64_t off = (uint64_t)(p - d0);
 		if (p >= d0 && off < (H->nsb << H->sb_log2)) {
 			uint32_t sb = (uint32_t)(off >> H->sb_log2);
-			uint32_t c1 = st_cls1(ld_rlx(state(H, sb)));
+			uint32_t c1 = st_cls1(ld(state(H, sb)));
 			if (c1 == 0) {
 				bump_stat(H, KMA_STAT_BAD_FREE);
 			} else {
@@ -257,6 +261,12 @@ __device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L
 		uint32_t sb = (uint32_t)(key / H->bm_words);
 		uint32_t wi = (uint32_t)(key % H->bm_words);
 		uint32_t mask = L.hmask[slot];
+		/* the barrier above waited for every thread's stores; a superblock
+		 * homed on another XCD also needs this XCD's L2 written back */
+		if (st_home(ld(state(H, sb))) != x) {
+			fadd(stats(H) + KMA_STAT_REMOTE_FREE, (uint64_t)L.hn[slot]);
+			release_l2();
+		}
 		uint32_t old = f_and(bitmap(H, sb) + wi, ~mask);
 		uint32_t n = __popc(old & mask);
 		uint32_t dbl = L.hn[slot] - n;
@@ -265,7 +275,7 @@ __device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L
 		if (n) {
 			uint64_t ns = fsub(state(H, sb), (uint64_t)n) - n;
 			if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
-				try_release(H, sb, ns, ctx);
+				try_release(H, sb, ns, x, ctx);
 		}
 	}
 	__syncthreads();

@@ -75,20 +75,28 @@ extern "C" int kmad_ref_init(void *heap, uint32_t log2, uint32_t sblocks)
 /* ------------------------------------------------------------------ */
 /* native layout: metadata init                                         */
 /* ------------------------------------------------------------------ */
-/* Pool shard s starts as the stack s, s+npool, s+2*npool, ... (lowest on
- * top); every other metadata word starts at zero (done by the host memset). */
+/* Every superblock starts in a clean shard: clean shard j (of the npool*7/8
+ * ... (S-1 per XCD) holds superblocks j, j+C, j+2C, ... (lowest on top), with
+ * C the number of clean shards; dirty shards start empty.  Every other
+ * metadata word starts at zero (done by the host memset). */
+__device__ __forceinline__ uint32_t clean_index_to_shard(uint32_t j, uint32_t S)
+{
+	return (j / (S - 1u)) * S + 1u + j % (S - 1u);
+}
+
 __global__ void k_nat_init(kma_dheap *H, uint64_t nsb, uint32_t npool)
 {
 	using namespace kma_nat;
 	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t S = npool / KMA_NXCD, C = npool - KMA_NXCD;
 	if (i < nsb) {
-		uint64_t nx = i + npool;
+		uint64_t nx = i + C;
 		*pnext(H, i) = nx < nsb ? (uint32_t)(nx + 1u) : 0u;
 	}
-	if (i < npool) {
-		uint64_t *hp = pool(H, (uint32_t)i);
+	if (i < C) {
+		uint64_t *hp = pool(H, clean_index_to_shard((uint32_t)i, S));
 		hp[0] = i < nsb ? (uint64_t)(i + 1u) : 0u;
-		hp[1] = i < nsb ? (nsb - i + npool - 1u) / npool : 0u;
+		hp[1] = i < nsb ? (nsb - i + C - 1u) / C : 0u;
 	}
 }
 
@@ -97,6 +105,8 @@ extern "C" int kmad_nat_init(void *heap, uint64_t nsb, uint32_t npool, uint64_t 
 	(void)nheads;
 	(void)npbwords;
 	uint64_t n = nsb > npool ? nsb : npool;
+	if (npool % KMA_NXCD || npool / KMA_NXCD < 2)
+		return -1;
 	uint32_t threads = 256;
 	uint64_t blocks = (n + threads - 1) / threads;
 	hipLaunchKernelGGL(k_nat_init, dim3((uint32_t)blocks), dim3(threads), 0, 0, (kma_dheap *)heap, nsb, npool);
@@ -113,19 +123,37 @@ __global__ void k_trim(kma_dheap *H, uint32_t nctx, uint32_t nclasses)
 	if (i >= nctx * nclasses)
 		return;
 	uint32_t ctx = i / nclasses, c = i % nclasses;
-	uint64_t h = __hip_atomic_exchange(head(H, ctx, c), HEAD_EMPTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+	uint64_t h = xchg(head(H, ctx, c), HEAD_EMPTY);
 	if (h <= HEAD_BUSY)
 		return;
 	uint32_t sb = (uint32_t)h - 2u;
 	uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
 	if (st_count(s) == 0 && st_cls1(s) != 0)
-		try_release(H, sb, s, ctx);
+		try_release(H, sb, s, st_home(s), ctx);
+}
+
+/* After a kernel boundary every L2 has been written back, so superblocks in
+ * dirty shards may go to any XCD: move them to their XCD's clean shards
+ * (one thread; no allocation may be in flight). */
+__global__ void k_clean_dirty(kma_dheap *H)
+{
+	using namespace kma_nat;
+	if (blockIdx.x != 0 || threadIdx.x != 0)
+		return;
+	for (uint32_t x = 0; x < KMA_NXCD; x++) {
+		uint32_t sb, n = 0;
+		while (pool_pop_shard(H, dirty_shard(H, x), &sb))
+			pool_push(H, clean_shard(H, x, n++), sb);
+	}
 }
 
 extern "C" int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses)
 {
 	uint32_t n = nctx * nclasses;
 	hipLaunchKernelGGL(k_trim, dim3((n + 255) / 256), dim3(256), 0, 0, (kma_dheap *)heap, nctx, nclasses);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	hipLaunchKernelGGL(k_clean_dirty, dim3(1), dim3(64), 0, 0, (kma_dheap *)heap);
 	KMA_CHECK(hipGetLastError());
 	KMA_CHECK(hipDeviceSynchronize());
 	return 0;
@@ -282,7 +310,7 @@ __global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, const uin
 	__syncthreads();
 	res_out[cs * 64 + l] = res[l];
 	for (uint32_t w = l; w < H->bm_words; w += 64)
-		bm_out[cs * H->bm_words + w] = kma_nat::ld_rlx(bm + w);
+		bm_out[cs * H->bm_words + w] = kma_nat::ld(bm + w);
 	if (l == 0)
 		done_out[cs] = done;
 }

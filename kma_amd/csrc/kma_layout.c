@@ -51,21 +51,23 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 		snprintf(err, errlen, "native layout: too many superblocks");
 		return -1;
 	}
+	/* contexts and pool shards are split evenly over the 8 XCDs; each XCD
+	 * has one dirty shard and at least one clean shard */
 	if (cfg->nctx == 0) {
 		uint64_t want = cfg->sblocks / (4ull * cfg->nclasses);
-		uint32_t n = 1;
+		uint32_t n = 8;
 		while ((uint64_t)n * 2 <= want && n < 1024)
 			n *= 2;
 		cfg->nctx = n;
 	}
-	if (!is_pow2(cfg->nctx) || cfg->nctx > 65536) {
-		snprintf(err, errlen, "native layout: nctx %u must be a power of two <= 65536", cfg->nctx);
+	if (!is_pow2(cfg->nctx) || cfg->nctx < 8 || cfg->nctx > 65536) {
+		snprintf(err, errlen, "native layout: nctx %u must be a power of two in 8..65536", cfg->nctx);
 		return -1;
 	}
 	if (cfg->npool == 0)
-		cfg->npool = cfg->nctx < 256 ? cfg->nctx : 256;
-	if (!is_pow2(cfg->npool) || cfg->npool > 65536) {
-		snprintf(err, errlen, "native layout: npool %u must be a power of two <= 65536", cfg->npool);
+		cfg->npool = cfg->nctx * 2 < 256 ? cfg->nctx * 2 : 256;
+	if (!is_pow2(cfg->npool) || cfg->npool < 16 || cfg->npool > 65536) {
+		snprintf(err, errlen, "native layout: npool %u must be a power of two in 16..65536", cfg->npool);
 		return -1;
 	}
 	if (cfg->page_log2 == 0)

@@ -148,7 +148,6 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 {
 	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	bool active = gid < nwi;
-	uint32_t ctx = blockIdx.x & (H->nctx - 1u);
 	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
 	for (uint32_t r = round0; r < round0 + nrounds; r++) {
 		uint32_t size = 0, hold = 0;
@@ -164,7 +163,7 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 		if (variant & 1u)
 			p = (volatile uint32_t *)kma_malloc(H, size);
 		else
-			p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, ctx, (variant >> 2) & 1u);
+			p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, (variant >> 2) & 1u);
 		if (active) {
 			mallocs++;
 			ops++;
@@ -191,7 +190,7 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 		if (variant & 2u)
 			kma_free(H, (void *)p);
 		else
-			kma_wg_free<KMA_BATCH_BLOCK>(H, (void *)p, L, ctx);
+			kma_wg_free<KMA_BATCH_BLOCK>(H, (void *)p, L);
 	}
 	wave_add_stats(stats, ops, nulls, bad, mallocs);
 }

@@ -25,15 +25,17 @@ dim3 g_sim_block_dim, g_sim_grid_dim;
 
 static void nat_init(kma_dheap *H)
 {
+	/* same distribution as k_nat_init: every superblock in a clean shard */
 	using namespace kma_nat;
+	uint32_t S = H->npool / KMA_NXCD, C = H->npool - KMA_NXCD;
 	for (uint64_t i = 0; i < H->nsb; i++) {
-		uint64_t nx = i + H->npool;
+		uint64_t nx = i + C;
 		*pnext(H, i) = nx < H->nsb ? (uint32_t)(nx + 1u) : 0u;
 	}
-	for (uint32_t i = 0; i < H->npool; i++) {
-		uint64_t *hp = pool(H, i);
-		hp[0] = i < H->nsb ? (uint64_t)(i + 1u) : 0u;
-		hp[1] = i < H->nsb ? (H->nsb - i + H->npool - 1u) / H->npool : 0u;
+	for (uint32_t j = 0; j < C; j++) {
+		uint64_t *hp = pool(H, (j / (S - 1u)) * S + 1u + j % (S - 1u));
+		hp[0] = j < H->nsb ? (uint64_t)(j + 1u) : 0u;
+		hp[1] = j < H->nsb ? (H->nsb - j + C - 1u) / C : 0u;
 	}
 }
 
@@ -48,8 +50,13 @@ static void trim(kma_dheap *H)
 			uint32_t sb = (uint32_t)h - 2u;
 			uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
 			if (st_count(s) == 0 && st_cls1(s) != 0)
-				try_release(H, sb, s, ctx);
+				try_release(H, sb, s, st_home(s), ctx);
 		}
+	for (uint32_t x = 0; x < KMA_NXCD; x++) {
+		uint32_t sb, n = 0;
+		while (pool_pop_shard(H, dirty_shard(H, x), &sb))
+			pool_push(H, clean_shard(H, x, n++), sb);
+	}
 }
 
 static int canonical_problems(kma_dheap *H)
@@ -57,7 +64,7 @@ static int canonical_problems(kma_dheap *H)
 	using namespace kma_nat;
 	int probs = 0;
 	for (uint64_t sb = 0; sb < H->nsb; sb++) {
-		if (*state(H, sb) & ((1ull << 40) - 1)) {
+		if (*state(H, sb) & ((1ull << 41) - 1)) {
 			if (probs < 5)
 				fprintf(stderr, "sb %llu state %#llx\n", (unsigned long long)sb,
 					(unsigned long long)*state(H, sb));
@@ -131,6 +138,7 @@ int main(int argc, char **argv)
 	std::vector<std::unique_ptr<sim_wave>> waves;
 	std::vector<std::unique_ptr<kma_wg_lds<KMA_BATCH_BLOCK>>> lds;
 	std::vector<std::thread> th;
+	g_sim_lane.bid = {0, 0, 0};
 	for (uint32_t b = 0; b < nwg; b++) {
 		wgs.emplace_back(new sim_wg(KMA_BATCH_BLOCK));
 		lds.emplace_back(new kma_wg_lds<KMA_BATCH_BLOCK>());

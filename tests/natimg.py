@@ -85,7 +85,7 @@ class NatHeap:
             probs.append("%d bitmap words nonzero" % int(np.count_nonzero(m["bitmap"])))
         if m["pbitmap"].any():
             probs.append("page bitmap nonzero")
-        gen_mask = np.uint64((1 << 40) - 1)
+        gen_mask = np.uint64((1 << 41) - 1)
         if (m["state"] & gen_mask).any():
             probs.append("%d state words not free" % int(np.count_nonzero(m["state"] & gen_mask)))
         if m["heads"].any():

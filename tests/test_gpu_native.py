@@ -33,7 +33,7 @@ def test_serial_trace_invariants(seed):
     sizes = [1, 7, 8, 9, 16, 24, 64, 100, 128, 500, 512, 1000, 2048, 4000, 8192, 8193,
              16384, 20000, 65536]
     ops, live = refimg.random_trace(rng, 4000, sizes, max_live=600)
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=4, npages=512) as h:
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=8, npages=512) as h:
         _say("native serial invariants seed=%d" % seed)
         out = h.trace_serial(ops)
         nh = natimg.NatHeap(h)
@@ -41,7 +41,7 @@ def test_serial_trace_invariants(seed):
         sizes_live = [int(ops[j]) for j in live]
         assert sum(1 for o, s in zip(offs, sizes_live) if o == 0 and s > 0) == 0
         assert natimg.check_blocks(nh, offs, sizes_live) == []
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=4, npages=512) as h:
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=8, npages=512) as h:
         h.trace_serial(refimg.free_all(ops, live))
         h.trim()
         nh = natimg.NatHeap(h)
@@ -50,7 +50,7 @@ def test_serial_trace_invariants(seed):
 
 
 def test_serial_size_zero_and_oversize():
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=1, npages=64) as h:
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=8, npages=64) as h:
         out = h.trace_serial([0, 8, -2, -1, (64 << 14) + 1, 1 << 20, -6])
         assert out[0] == 0 and out[1] != 0 and out[4] == 0
         assert out[5] != 0  # 64 pages of 16 KiB: the whole page region
@@ -59,7 +59,7 @@ def test_serial_size_zero_and_oversize():
 
 
 def test_double_free_detected_not_applied():
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=1) as h:
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=8) as h:
         out = h.trace_serial([64, 64, -1, -1, -2])
         assert out[0] and out[1]
         nh = natimg.NatHeap(h)
@@ -69,7 +69,7 @@ def test_double_free_detected_not_applied():
 
 
 def test_exhaustion_returns_null_then_recovers():
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 8, nctx=1, npool=1) as h:
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 8, nctx=8, npool=16) as h:
         # 8 superblocks of 2 x 8 KiB blocks
         ops = [8192] * 20
         frees = [-(i + 1) for i in range(20)]
@@ -158,7 +158,7 @@ def _expected_claim(bm, cap, base, got, d0, size):
 
 def test_wave_claim_matches_model():
     rng = np.random.default_rng(11)
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=1) as h:
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 256, nctx=8) as h:
         nh = natimg.NatHeap(h)
         W = nh.bm_words
         cls, base, got, bms = [], [], [], []
