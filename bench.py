@@ -86,18 +86,26 @@ def main():
                 % (name, st["ops_per_s"], st["gpu_ms"] / K, st["nulls"], st["bad"], st["double_frees"],
                    st["bad_frees"], st["pool_empty"], st["waits"]))
 
-    import ctypes
+    # malloc latency and phase breakdown: separate instrumented runs, so the
+    # timed runs above carry no probes
+    lat, phases = {}, {}
+    for form, name in ((kma_amd.FORM_PLAIN, "plain"), (kma_amd.FORM_BATCHED, "batched")):
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS) as h:
+            st = h.run(kma_amd.WL_LOGUNI, NWI, 5, form=form, block=BLOCK, seed=SEED, warmup=W,
+                       variant=kma_amd.VAR_LATENCY)
+            lat[name] = {"p50_ns": st["lat_p50_ns"], "p99_ns": st["lat_p99_ns"], "max_ns": st["lat_max_ns"]}
+            st = h.run(kma_amd.WL_LOGUNI, NWI, 5, form=form, block=BLOCK, seed=SEED, round0=100,
+                       variant=kma_amd.VAR_PHASES)
+            phases[name] = dict(zip(("malloc_ns", "hold_ns", "free_ns"), st["phase_ns"]))
+            log("bench: %s malloc latency p50 %.0f ns p99 %.0f ns max %.0f ns; phases per wave-round %s"
+                % (name, lat[name]["p50_ns"], lat[name]["p99_ns"], lat[name]["max_ns"],
+                   {k: round(v) for k, v in phases[name].items()}))
 
     peak = {}
     for pat, name in ((0, "distinct"), (2, "same_word_per_wave")):
-        ops = ctypes.c_double()
-        ms = ctypes.c_double()
-        rc = kma_amd.lib().kma_atomic_peak(ctypes.c_uint64(1 << 30), ctypes.c_uint64(NWI), 64, pat, 2, 5,
-                                           ctypes.byref(ops), ctypes.byref(ms))
-        if rc != 0:
-            raise SystemExit("atomic peak: " + kma_amd.last_error())
-        peak[name] = ops.value
-        log("bench: atomicOr peak (%s) %.3e /s" % (name, ops.value))
+        ops, _ = kma_amd.atomic_peak(1 << 30, NWI, 64, pat)
+        peak[name] = ops
+        log("bench: atomicOr peak (%s) %.3e /s" % (name, ops))
 
     b, p = res["batched"], res["plain"]
     value = b["ops_per_s"]
@@ -128,6 +136,8 @@ def main():
         "plain": {"value": p["ops_per_s"], "ms_per_step": p["gpu_ms"] / K, "launches": p["launches"]},
         "fused_per_lane": {"value": res["fused"]["ops_per_s"], "ms_per_step": res["fused"]["gpu_ms"] / K},
         "batched_over_plain": value / p["ops_per_s"],
+        "malloc_latency": lat,
+        "phases_per_wave_round": phases,
         "invariants": {k: int(sum(r[k] for r in res.values())) for k in ("nulls", "bad", "double_frees", "bad_frees")},
         "roofline": {
             "bound": "hbm",

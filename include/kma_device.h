@@ -189,11 +189,15 @@ __device__ __forceinline__ T f_and(T *p, T v)
 	return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* Wait until this thread's earlier memory operations have completed
- * (workgroup-scope release: s_waitcnt only, no cache maintenance). */
+/* Wait until this wavefront's earlier memory operations have completed
+ * (s_waitcnt vmcnt(0) expcnt(0) lgkmcnt(0)), with compiler barriers so no
+ * access moves across it.  No cache maintenance: a completed store is in
+ * this XCD's L2 and a completed atomic is performed at the coherent level. */
 __device__ __forceinline__ void order_prior()
 {
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+	__atomic_signal_fence(__ATOMIC_SEQ_CST);
+	__builtin_amdgcn_s_waitcnt(0);
+	__atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 /* Agent-scope release: also writes back this XCD's L2. */
 __device__ __forceinline__ void release_l2()

@@ -215,6 +215,9 @@ __device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L
 	const uint32_t tid = threadIdx.x, nth = blockDim.x;
 	const uint32_t x = xcc(), ctx = wg_ctx_of(H, x);
 	const uint32_t HN = 2u * BLOCK;
+	/* every thread's stores to its block complete before any bit of the
+	 * batch is released (the barrier below only orders LDS) */
+	order_prior();
 	for (uint32_t i = tid; i < HN; i += nth) {
 		L.hkey[i] = 0;
 		L.hmask[i] = 0;

@@ -39,9 +39,15 @@ typedef struct kma_workload {
 	uint64_t seed;
 	uint32_t warmup;  /* untimed rounds run first */
 	uint32_t reps;    /* reserved, 0 */
-	uint32_t variant; /* BATCHED only: bit 0 per-lane kma_malloc, bit 1 per-lane
-			     kma_free, in place of the work-group collective calls */
+	uint32_t variant; /* KMA_VAR_* bits */
 } kma_workload;
+
+/* Workload variants. */
+#define KMA_VAR_LANE_MALLOC 0x1u   /* BATCHED: per-lane kma_malloc instead of kma_wg_malloc */
+#define KMA_VAR_LANE_FREE   0x2u   /* BATCHED: per-lane kma_free instead of kma_wg_free */
+#define KMA_VAR_LANE_CLAIM  0x4u   /* BATCHED: one bitmap atomic per block, not per word */
+#define KMA_VAR_LATENCY     0x100u /* record every malloc's latency (wall clock) */
+#define KMA_VAR_PHASES      0x200u /* time the malloc / hold / free phases */
 
 typedef struct kma_run_stats {
 	double gpu_ms;          /* HIP-event time of the timed region */
@@ -55,6 +61,12 @@ typedef struct kma_run_stats {
 	uint64_t bad_frees;     /* heap counter (native layout) */
 	uint64_t pool_empty;    /* refills that found no free superblock */
 	uint64_t waits[4];      /* watchdog expiries: reserve, claim, wavefront claim, pool */
+	uint64_t remote_frees;  /* frees issued from a non-home XCD */
+	double lat_p50_ns;      /* KMA_VAR_LATENCY: malloc latency percentiles */
+	double lat_p99_ns;
+	double lat_max_ns;
+	double phase_ns[3];     /* KMA_VAR_PHASES: malloc / hold / free time per
+				   wavefront-round */
 } kma_run_stats;
 
 /* Run a per-work-item workload on the heap. */

@@ -19,6 +19,8 @@ MAX_CLASSES = 24
 
 WL_FIXED, WL_LOGUNI, WL_REF_TEST, WL_LOWVAR, WL_HIGHVAR = range(5)
 FORM_PLAIN, FORM_BATCHED, FORM_FUSED = 0, 1, 2
+VAR_LANE_MALLOC, VAR_LANE_FREE, VAR_LANE_CLAIM = 0x1, 0x2, 0x4
+VAR_LATENCY, VAR_PHASES = 0x100, 0x200
 
 # Exported symbols, as declared in include/*.h (checked by the CPU tests).
 EXPORTS = [
@@ -72,11 +74,17 @@ class RunStats(ctypes.Structure):
         ("bad_frees", ctypes.c_uint64),
         ("pool_empty", ctypes.c_uint64),
         ("waits", ctypes.c_uint64 * 4),
+        ("remote_frees", ctypes.c_uint64),
+        ("lat_p50_ns", ctypes.c_double),
+        ("lat_p99_ns", ctypes.c_double),
+        ("lat_max_ns", ctypes.c_double),
+        ("phase_ns", ctypes.c_double * 3),
     ]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["waits"] = list(self.waits)
+        d["phase_ns"] = list(self.phase_ns)
         return d
 
 

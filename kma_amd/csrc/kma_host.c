@@ -303,6 +303,33 @@ static int read_native_stats(kma_heap *h, uint64_t *st)
 	return 0;
 }
 
+static int cmp_u32(const void *a, const void *b)
+{
+	uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+	return x < y ? -1 : x > y;
+}
+
+/* nanoseconds per tick of the device's constant wall clock */
+static double wall_clock_ns(void)
+{
+	int dev = 0, khz = 0;
+	if (hipGetDevice(&dev) != hipSuccess ||
+	    hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+		return 10.0; /* 100 MHz */
+	return 1e6 / (double)khz;
+}
+
+static int launch_form(kma_heap *h, const kma_workload *wl, uint32_t form, uint32_t round0, uint32_t nrounds,
+		       uint32_t variant, unsigned long long *d_stats, uint32_t *d_lat, hipStream_t s)
+{
+	int layout = h->cfg.layout == KMA_LAYOUT_REFERENCE ? 0 : 1;
+	if (form == KMA_FORM_BATCHED)
+		return kmad_batched(h->dev, round0, nrounds, (int)wl->mode, wl->fixed, wl->seed, wl->nwi, variant,
+				    d_stats, d_lat, s);
+	return kmad_rounds(layout, h->dev, round0, nrounds, (int)wl->mode, wl->fixed, wl->seed, wl->nwi, wl->block,
+			   variant, d_stats, d_lat, s);
+}
+
 int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 {
 	if (!h || !wl || !out) {
@@ -310,64 +337,58 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		return -1;
 	}
 	memset(out, 0, sizeof(*out));
-	if (wl->mode > KMA_WL_HIGHVAR || wl->form > KMA_FORM_FUSED || wl->nwi == 0 || wl->block == 0 || (wl->block & 63) || wl->block > 256) {
+	if (wl->mode > KMA_WL_HIGHVAR || wl->form > KMA_FORM_FUSED || wl->nwi == 0 || wl->block == 0 ||
+	    (wl->block & 63) || wl->block > 256 || wl->rounds == 0) {
 		set_err("bad workload");
 		return -1;
 	}
-	int layout = h->cfg.layout == KMA_LAYOUT_REFERENCE ? 0 : 1;
+	if (wl->form == KMA_FORM_BATCHED && h->cfg.layout != KMA_LAYOUT_NATIVE) {
+		set_err("the batched form needs a native-layout heap");
+		return -1;
+	}
 	unsigned long long *d_stats = NULL;
+	uint32_t *d_lat = NULL, *h_lat = NULL;
 	hipStream_t s = NULL;
 	hipEvent_t e0 = NULL, e1 = NULL;
-	int rc = -1;
-	uint64_t st0[NAT_NSTATS], st1[NAT_NSTATS];
-	unsigned long long hs[4];
 	hipGraph_t graph = NULL;
 	hipGraphExec_t gexec = NULL;
-	if (read_native_stats(h, st0) != 0)
-		return -1;
-	if (hipMalloc((void **)&d_stats, 4 * 8) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
-	    hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+	int rc = -1;
+	uint64_t st0[NAT_NSTATS], st1[NAT_NSTATS];
+	unsigned long long hs[8];
+	uint64_t nlat = (wl->variant & KMA_VAR_LATENCY) ? wl->nwi * wl->rounds : 0;
+	uint32_t untimed = wl->variant & ~(KMA_VAR_LATENCY | KMA_VAR_PHASES);
+	if (hipMalloc((void **)&d_stats, sizeof(hs)) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
+	    hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+	    (nlat && hipMalloc((void **)&d_lat, nlat * 4) != hipSuccess)) {
 		set_err("workload setup failed");
 		goto done;
 	}
-	if (wl->form == KMA_FORM_BATCHED && layout != 1) {
-		set_err("the batched form needs a native-layout heap");
+	if (hipMemset(d_stats, 0, sizeof(hs)) != hipSuccess)
 		goto done;
-	}
 	/* warm-up rounds, untimed, not counted */
-	if (hipMemset(d_stats, 0, 32) != hipSuccess)
-		goto done;
 	if (wl->warmup) {
-		if (wl->form == KMA_FORM_BATCHED)
-			rc = kmad_batched(h->dev, wl->round0, wl->warmup, (int)wl->mode, wl->fixed, wl->seed, wl->nwi,
-					  wl->variant, d_stats, s);
-		else
-			rc = kmad_rounds(layout, h->dev, wl->round0, wl->warmup, (int)wl->mode, wl->fixed, wl->seed,
-					 wl->nwi, wl->block, d_stats, s);
+		rc = launch_form(h, wl, wl->form == KMA_FORM_PLAIN ? KMA_FORM_FUSED : wl->form, wl->round0,
+				 wl->warmup, untimed, d_stats, NULL, s);
 		if (rc) {
 			set_err("warm-up launch failed: %d", rc);
 			goto done;
 		}
 	}
-	if (hipStreamSynchronize(s) != hipSuccess || hipMemset(d_stats, 0, 32) != hipSuccess) {
+	if (hipStreamSynchronize(s) != hipSuccess || hipMemset(d_stats, 0, sizeof(hs)) != hipSuccess ||
+	    read_native_stats(h, st0) != 0) {
 		set_err("warm-up failed");
-		rc = -1;
-		goto done;
-	}
-	if (read_native_stats(h, st0) != 0) {
 		rc = -1;
 		goto done;
 	}
 	uint32_t r0 = wl->round0 + wl->warmup;
 	if (wl->form == KMA_FORM_BATCHED) {
 		/* one persistent launch for all rounds, captured in a graph */
-		if (hipStreamBeginCapture(s, hipStreamCaptureModeGlobal) != hipSuccess) {
+		if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
 			set_err("graph capture failed");
 			rc = -1;
 			goto done;
 		}
-		rc = kmad_batched(h->dev, r0, wl->rounds, (int)wl->mode, wl->fixed, wl->seed, wl->nwi, wl->variant, d_stats,
-				  s);
+		rc = launch_form(h, wl, KMA_FORM_BATCHED, r0, wl->rounds, wl->variant, d_stats, d_lat, s);
 		if (hipStreamEndCapture(s, &graph) != hipSuccess || rc != 0 ||
 		    hipGraphInstantiate(&gexec, graph, NULL, NULL, 0) != hipSuccess) {
 			set_err("graph build failed: %d", rc);
@@ -378,8 +399,9 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	hipEventRecord(e0, s);
 	if (wl->form == KMA_FORM_PLAIN) {
 		for (uint32_t r = 0; r < wl->rounds; r++) {
-			rc = kmad_rounds(layout, h->dev, r0 + r, 1, (int)wl->mode, wl->fixed, wl->seed, wl->nwi,
-					 wl->block, d_stats, s);
+			rc = kmad_rounds(h->cfg.layout == KMA_LAYOUT_REFERENCE ? 0 : 1, h->dev, r0 + r, 1, (int)wl->mode,
+					 wl->fixed, wl->seed, wl->nwi, wl->block, wl->variant, d_stats,
+					 d_lat ? d_lat + (uint64_t)r * wl->nwi : NULL, s);
 			if (rc) {
 				set_err("launch failed: %d", rc);
 				goto done;
@@ -394,8 +416,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		}
 		out->launches = 1;
 	} else {
-		rc = kmad_rounds(layout, h->dev, r0, wl->rounds, (int)wl->mode, wl->fixed, wl->seed, wl->nwi,
-				 wl->block, d_stats, s);
+		rc = launch_form(h, wl, KMA_FORM_FUSED, r0, wl->rounds, wl->variant, d_stats, d_lat, s);
 		if (rc) {
 			set_err("launch failed: %d", rc);
 			goto done;
@@ -412,7 +433,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	hipEventElapsedTime(&ms, e0, e1);
 	out->gpu_ms = ms;
 	out->kernel_ms_avg = out->launches ? ms / (double)out->launches : 0;
-	if (hipMemcpy(hs, d_stats, 32, hipMemcpyDeviceToHost) != hipSuccess || read_native_stats(h, st1) != 0) {
+	if (hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess || read_native_stats(h, st1) != 0) {
 		set_err("stats download failed");
 		rc = -1;
 		goto done;
@@ -426,8 +447,26 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	out->pool_empty = st1[2] - st0[2];
 	for (int k = 0; k < 4; k++)
 		out->waits[k] = st1[3 + k] - st0[3 + k];
+	out->remote_frees = st1[7] - st0[7];
+	double tick = wall_clock_ns();
+	if (hs[7])
+		for (int k = 0; k < 3; k++)
+			out->phase_ns[k] = tick * (double)hs[4 + k] / (double)hs[7];
+	if (nlat) {
+		h_lat = (uint32_t *)malloc(nlat * 4);
+		if (!h_lat || hipMemcpy(h_lat, d_lat, nlat * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+			set_err("latency download failed");
+			rc = -1;
+			goto done;
+		}
+		qsort(h_lat, nlat, 4, cmp_u32);
+		out->lat_p50_ns = tick * h_lat[nlat / 2];
+		out->lat_p99_ns = tick * h_lat[(nlat * 99) / 100];
+		out->lat_max_ns = tick * h_lat[nlat - 1];
+	}
 	rc = 0;
 done:
+	free(h_lat);
 	if (gexec)
 		hipGraphExecDestroy(gexec);
 	if (graph)
@@ -438,6 +477,8 @@ done:
 		hipEventDestroy(e1);
 	if (s)
 		hipStreamDestroy(s);
+	if (d_lat)
+		hipFree(d_lat);
 	if (d_stats)
 		hipFree(d_stats);
 	return rc;

@@ -199,26 +199,25 @@ extern "C" int kmad_serial(int layout, void *heap, const int64_t *d_ops, uint64_
 /* plain per-launch workloads                                           */
 /* ------------------------------------------------------------------ */
 template <class HeapT>
-__global__ void __launch_bounds__(256) k_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds, int mode,
-						uint32_t fixed, uint64_t seed, uint64_t nwi,
-						unsigned long long *stats)
+__global__ void __launch_bounds__(256) k_rounds(HeapT *heap, kma_wl_args a)
 {
-	wl_rounds<HeapT>(heap, round0, nrounds, mode, fixed, seed, nwi, stats);
+	wl_rounds<HeapT>(heap, a);
 }
 
 extern "C" int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed,
-			   uint64_t seed, uint64_t nwi, uint32_t block, unsigned long long *d_stats, void *stream)
+			   uint64_t seed, uint64_t nwi, uint32_t block, uint32_t variant, unsigned long long *d_stats,
+			   uint32_t *d_lat, void *stream)
 {
 	if (block == 0 || block > 256 || (block & 63))
 		return -1;
 	uint64_t blocks = (nwi + block - 1) / block;
 	hipStream_t s = (hipStream_t)stream;
+	kma_wl_args a = {round0, nrounds, mode, fixed, seed, nwi, variant, d_stats, d_lat};
 	if (layout == 0)
-		hipLaunchKernelGGL(k_rounds<clheap>, dim3((uint32_t)blocks), dim3(block), 0, s, (clheap *)heap,
-				   round0, nrounds, mode, fixed, seed, nwi, d_stats);
+		hipLaunchKernelGGL(k_rounds<clheap>, dim3((uint32_t)blocks), dim3(block), 0, s, (clheap *)heap, a);
 	else
-		hipLaunchKernelGGL(k_rounds<kma_dheap>, dim3((uint32_t)blocks), dim3(block), 0, s,
-				   (kma_dheap *)heap, round0, nrounds, mode, fixed, seed, nwi, d_stats);
+		hipLaunchKernelGGL(k_rounds<kma_dheap>, dim3((uint32_t)blocks), dim3(block), 0, s, (kma_dheap *)heap,
+				   a);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }
@@ -271,20 +270,19 @@ extern "C" int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, u
  * and allocates / frees them collectively (kma_wg_device.h).  Work-items
  * past nwi take part with empty requests so control flow stays uniform.
  */
-__global__ void __launch_bounds__(KMA_BATCH_BLOCK) k_batched(kma_dheap *H, uint32_t round0, uint32_t nrounds, int mode,
-							     uint32_t fixed, uint64_t seed, uint64_t nwi,
-							     uint32_t variant, unsigned long long *stats)
+__global__ void __launch_bounds__(KMA_BATCH_BLOCK) k_batched(kma_dheap *H, kma_wl_args a)
 {
 	__shared__ kma_wg_lds<KMA_BATCH_BLOCK> L;
-	wl_batched(H, L, round0, nrounds, mode, fixed, seed, nwi, variant, stats);
+	wl_batched(H, L, a);
 }
 
 extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed, uint64_t seed,
-			    uint64_t nwi, uint32_t variant, unsigned long long *d_stats, void *stream)
+			    uint64_t nwi, uint32_t variant, unsigned long long *d_stats, uint32_t *d_lat, void *stream)
 {
 	uint64_t blocks = (nwi + KMA_BATCH_BLOCK - 1) / KMA_BATCH_BLOCK;
+	kma_wl_args a = {round0, nrounds, mode, fixed, seed, nwi, variant, d_stats, d_lat};
 	hipLaunchKernelGGL(k_batched, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, (hipStream_t)stream,
-			   (kma_dheap *)heap, round0, nrounds, mode, fixed, seed, nwi, variant, d_stats);
+			   (kma_dheap *)heap, a);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }

@@ -26,11 +26,12 @@ int kmad_serial(int layout, void *heap, const int64_t *d_ops, uint64_t n,
 /* Workload kernels: see kma_work.h for the parameters. */
 int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds,
 		int mode, uint32_t fixed, uint64_t seed, uint64_t nwi,
-		uint32_t block, unsigned long long *d_stats, void *stream);
+		uint32_t block, uint32_t variant, unsigned long long *d_stats,
+		uint32_t *d_lat, void *stream);
 
 int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode,
 		 uint32_t fixed, uint64_t seed, uint64_t nwi, uint32_t variant,
-		 unsigned long long *d_stats, void *stream);
+		 unsigned long long *d_stats, uint32_t *d_lat, void *stream);
 
 int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses);
 

@@ -52,13 +52,60 @@ __device__ __forceinline__ void wave_add_stats(unsigned long long *stats, uint64
  * stats: [0] heap ops issued, [1] NULL returns, [2] read-back mismatches,
  *        [3] mallocs issued.
  */
+/* Launch arguments shared by the workload kernels. */
+struct kma_wl_args {
+	uint32_t round0, nrounds;
+	int mode;
+	uint32_t fixed;
+	uint64_t seed;
+	uint64_t nwi;
+	uint32_t variant;          /* KMA_VAR_* bits */
+	unsigned long long *stats; /* [0] ops [1] nulls [2] bad [3] mallocs
+				      [4..6] malloc/hold/free ticks summed over
+				      wavefronts and rounds, [7] samples */
+	uint32_t *lat;             /* optional: malloc ticks per (round, work-item) */
+};
+
+/* Per-wavefront phase clock (lane 0's view; the wavefront runs in step). */
+struct kma_phase {
+	uint64_t t = 0, acc[3] = {0, 0, 0}, n = 0;
+	bool on;
+	__device__ kma_phase(bool enable) : on(enable) {}
+	__device__ __forceinline__ void start()
+	{
+		if (on)
+			t = wall_clock64();
+	}
+	__device__ __forceinline__ void mark(int k)
+	{
+		if (on) {
+			uint64_t u = wall_clock64();
+			acc[k] += u - t;
+			t = u;
+			n += (k == 2);
+		}
+	}
+	__device__ __forceinline__ void flush(unsigned long long *stats)
+	{
+		if (on && __lane_id() == 0) {
+			for (int k = 0; k < 3; k++)
+				atomicAdd(stats + 4 + k, (unsigned long long)acc[k]);
+			atomicAdd(stats + 7, (unsigned long long)n);
+		}
+	}
+};
+
 template <class HeapT>
-__device__ inline void wl_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed,
-				 uint64_t seed, uint64_t nwi, unsigned long long *stats)
+__device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 {
+	const uint32_t round0 = a.round0, nrounds = a.nrounds, fixed = a.fixed;
+	const int mode = a.mode;
+	const uint64_t seed = a.seed, nwi = a.nwi;
+	unsigned long long *stats = a.stats;
 	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
 	bool alive = gid < nwi;
+	kma_phase ph((a.variant & KMA_VAR_PHASES) != 0);
 	for (uint32_t r = round0; r < round0 + nrounds; r++) {
 		if (!alive)
 			break;
@@ -70,7 +117,12 @@ __device__ inline void wl_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds,
 				size = 1u << (3 + kma_mix(seed, gid, r, 0) % 9);
 				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
 			}
+			ph.start();
+			uint64_t t0 = a.lat ? wall_clock64() : 0;
 			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, size);
+			if (a.lat)
+				a.lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
+			ph.mark(0);
 			mallocs++;
 			ops++;
 			if (!p) {
@@ -91,7 +143,9 @@ __device__ inline void wl_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds,
 				if (p[0] != pat)
 					bad++;
 			}
+			ph.mark(1);
 			kma_free(heap, (void *)p);
+			ph.mark(2);
 			ops++;
 		} else if (mode == KMA_WL_REF_TEST) {
 			volatile uint64_t *p = (volatile uint64_t *)kma_malloc(heap, 8);
@@ -138,14 +192,18 @@ __device__ inline void wl_rounds(HeapT *heap, uint32_t round0, uint32_t nrounds,
 		}
 	}
 	wave_add_stats(stats, ops, nulls, bad, mallocs);
+	ph.flush(stats);
 }
 
 #define KMA_BATCH_BLOCK 256
 
-__device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, uint32_t round0, uint32_t nrounds,
-				  int mode, uint32_t fixed, uint64_t seed, uint64_t nwi, uint32_t variant,
-				  unsigned long long *stats)
+__device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, const kma_wl_args &a)
 {
+	const uint32_t round0 = a.round0, nrounds = a.nrounds, fixed = a.fixed, variant = a.variant;
+	const int mode = a.mode;
+	const uint64_t seed = a.seed, nwi = a.nwi;
+	unsigned long long *stats = a.stats;
+	kma_phase ph((variant & KMA_VAR_PHASES) != 0);
 	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	bool active = gid < nwi;
 	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
@@ -160,10 +218,15 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 			}
 		}
 		volatile uint32_t *p;
+		ph.start();
+		uint64_t t0 = a.lat ? wall_clock64() : 0;
 		if (variant & 1u)
 			p = (volatile uint32_t *)kma_malloc(H, size);
 		else
 			p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, (variant >> 2) & 1u);
+		if (a.lat && active)
+			a.lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
+		ph.mark(0);
 		if (active) {
 			mallocs++;
 			ops++;
@@ -187,12 +250,15 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 				ops++;
 			}
 		}
+		ph.mark(1);
 		if (variant & 2u)
 			kma_free(H, (void *)p);
 		else
 			kma_wg_free<KMA_BATCH_BLOCK>(H, (void *)p, L);
+		ph.mark(2);
 	}
 	wave_add_stats(stats, ops, nulls, bad, mallocs);
+	ph.flush(stats);
 }
 
 

@@ -117,6 +117,7 @@ static inline uint64_t clock64() { return wall_clock64(); }
 #define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
 /* fences: sequentially consistent on the host */
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
+#define __builtin_amdgcn_s_waitcnt(n) __atomic_thread_fence(__ATOMIC_SEQ_CST)
 /* hwreg reads: only HW_REG_XCC_ID is used; work-groups are spread over 8
  * simulated XCDs round-robin, as the hardware dispatcher does */
 #define __builtin_amdgcn_s_getreg(enc) ((int)(g_sim_lane.bid.x % 8u))

@@ -133,7 +133,7 @@ int main(int argc, char **argv)
 
 	g_sim_block_dim = {KMA_BATCH_BLOCK, 1, 1};
 	g_sim_grid_dim = {nwg, 1, 1};
-	unsigned long long stats[4] = {0, 0, 0, 0};
+	unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	std::vector<std::unique_ptr<sim_wg>> wgs;
 	std::vector<std::unique_ptr<sim_wave>> waves;
 	std::vector<std::unique_ptr<kma_wg_lds<KMA_BATCH_BLOCK>>> lds;
@@ -152,7 +152,8 @@ int main(int argc, char **argv)
 				g_sim_lane.bid = {b, 0, 0};
 				g_sim_lane.wg = wgs[b].get();
 				g_sim_lane.wave = waves[b * (KMA_BATCH_BLOCK / 64) + t / 64].get();
-				wl_batched(H, *lds[b], 0, rounds, mode, 64, seed, nwi, 0, stats);
+				kma_wl_args wa = {0, rounds, mode, 64, seed, nwi, 0, stats, nullptr};
+				wl_batched(H, *lds[b], wa);
 			});
 	for (auto &t : th)
 		t.join();
