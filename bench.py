@@ -10,6 +10,11 @@ mallocs 2^e bytes (e uniform in 3..11, splitmix64 of seed 1), holds it for
 One step = one round = 1,048,576 heap ops.  Inputs are generated on the
 device from the seed; nothing crosses PCIe in the timed region.
 
+Also measured (key "batched_stream"): the batched-stream workload, a
+host-generated trace of 4,096 uneven batches already resident in HBM, run
+as one launch per batch (plain) and as one graph-captured persistent launch
+with the autotuned heap.
+
 Prints ONE JSON line (rank 0, N = 1).
 """
 import argparse
@@ -56,12 +61,77 @@ def cpu_baseline(rounds=4):
     }
 
 
+# Batched-stream workload: 4,096 batches, Zipf(1.2) sizes over 1..65,536
+# requests, 40 % frees of live blocks of earlier batches, sizes log-uniform
+# 16 B .. 64 KiB (>= 16 KiB on pages), 4 GiB heap, seed 3, drained at the end.
+STREAM = dict(nbatch=4096, max_batch=65536, zipf=1.2, free_pct=40, min_size=16, max_size=65536,
+              page_min=16384, seed=3)
+STREAM_HEAP = 4 << 30
+
+
+def _stream_line(st):
+    good = st["mallocs"] + st["frees"]
+    return {
+        "ops_per_s": st["ops"] / (st["gpu_ms"] * 1e-3),
+        "successful_ops_per_s": good / (st["gpu_ms"] * 1e-3),
+        "ms_per_replay": st["gpu_ms"],
+        "launches": st["launches"],
+        "ops": st["ops"],
+        "nulls": st["nulls"],
+        "large_nulls": st["large_nulls"],
+        "violations": int(st["bad"] + st["wait_timeouts"] + st["double_frees"] + st["bad_frees"] + sum(st["waits"])),
+    }
+
+
+def bench_stream(reps):
+    """Per-launch form (one launch per batch) on the default heap, then the
+    autotuned persistent form (one graph launch per replay)."""
+    import kma_amd
+
+    s = kma_amd.Stream(**STREAM)
+    log("bench: stream of %d requests in %d batches" % (s.info["nreq"], s.info["nbatch"]))
+    cfg = s.heap_config(STREAM_HEAP, 14, kma_amd.TABLE_POW2, 14)
+    with kma_amd.Heap.from_config(cfg) as h:
+        plain = _stream_line(s.run(h, form=kma_amd.FORM_PLAIN, warmup=1, reps=max(1, reps // 2)))
+        log("bench: stream plain %s" % plain)
+        untuned = _stream_line(s.run(h, form=kma_amd.FORM_BATCHED, nwg=1024, per_thread=1, warmup=1, reps=reps))
+        log("bench: stream persistent, default heap %s" % untuned)
+    limit = int(plain["nulls"] * 1.05) + 1
+    tune = s.autotune(STREAM_HEAP, 14, null_limit=limit, reps=2)
+    log("bench: autotuner picked sb_log2 %d table %d nwg %d per_thread %d (%.3e ops/s, nulls %d) of %d trials"
+        % (tune["config"]["sb_log2"], tune["table"], tune["nwg"], tune["per_thread"], tune["ops_per_s"],
+           tune["nulls"], len(tune["trials"])))
+    with kma_amd.Heap.from_config(tune["cfg"]) as h:
+        batched = _stream_line(s.run(h, form=kma_amd.FORM_BATCHED, nwg=tune["nwg"],
+                                     per_thread=tune["per_thread"], warmup=1, reps=reps))
+    log("bench: stream persistent, tuned %s" % batched)
+    return {
+        "workload": "4,096 batches, Zipf(1.2) over 1..65,536 requests, 40% frees of earlier batches' "
+                    "live blocks, sizes log-uniform 16 B..64 KiB (>16 KiB on 16 KiB pages), 4 GiB heap, "
+                    "seed 3, drain batch at the end",
+        "requests": s.info["nreq"],
+        "batches": s.info["nbatch"],
+        "peak_live_demand_bytes": s.info["peak_small_bytes"] + s.info["peak_large_bytes"],
+        "plain": plain,
+        "persistent_default_heap": untuned,
+        "persistent_tuned": batched,
+        "batched_over_plain": batched["ops_per_s"] / plain["ops_per_s"],
+        "null_limit": limit,
+        "tuned": {"sb_log2": tune["config"]["sb_log2"], "classes": tune["config"]["classes"],
+                  "sblocks": tune["config"]["sblocks"], "npages": tune["config"]["npages"],
+                  "nwg": tune["nwg"], "per_thread": tune["per_thread"]},
+        "trials": [{k: t[k] for k in ("sb_log2", "table", "nwg", "per_thread", "ops_per_s", "nulls", "violations")}
+                   for t in tune["trials"]],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every heap is device-local and this build runs on "
@@ -154,6 +224,8 @@ def main():
             "frac": value / peak["distinct"],
         },
     }
+    if not args.no_stream:
+        out["batched_stream"] = bench_stream(5)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -50,8 +50,8 @@
  * Context heads (64-bit): 0 empty, 1 refill in progress, otherwise
  * (gen << 32) | (sb + 2).  Pool shards are Treiber stacks with a 64-bit
  * {tag, top} head (ABA tag bumped on every pop and push); XCD x owns shards
- * [x*S, x*S + S) with S = npool/8: shard x*S is its dirty shard, the rest are
- * clean.  Large requests (> largest class) take 1..64 contiguous pages from a
+ * [x*S, x*S + S) with S = npool/8: the first S/2 are its dirty shards, the
+ * rest are clean.  Large requests (> largest class) take 1..64 contiguous pages from a
  * page bitmap (64-bit words, runs aligned to the next power of two).
  */
 #ifndef KMA_DEVICE_H
@@ -92,6 +92,7 @@ typedef struct kma_dheap {
 	uint32_t class_cap[32];
 	uint64_t class_magic[32]; /* ceil(2^40 / size): exact block index */
 	uint8_t class_for_log2[64];
+	uint64_t off_psum; /* page-window summaries, KMA_PAGE_LEVELS levels */
 } kma_dheap;
 
 /* Per-heap counters (u64 at off_stats). */
@@ -104,6 +105,13 @@ typedef struct kma_dheap {
 #define KMA_STAT_WAIT_POOL 6    /* a pool push/pop hit the watchdog */
 #define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
 #define KMA_NSTATS 8
+/* The stats block is 256 B of counters followed by one 128-B line per XCD
+ * holding that XCD's pooled-superblock count (an upper bound: it is raised
+ * before a push publishes and lowered after a pop succeeds). */
+#define KMA_STATS_BYTES (256u + 128u * 8u)
+/* Page runs of k pages use an aligned window of 2^l >= k pages inside one
+ * 64-page bitmap word; levels l = 0..6. */
+#define KMA_PAGE_LEVELS 7u
 
 /* Every wait in the native allocator gives up after this many ticks of the
  * constant 100 MHz wall clock (0.25 s).  Reaching it means the heap is
@@ -238,6 +246,9 @@ __device__ __forceinline__ uint64_t *pbits(const kma_dheap *H) { return (uint64_
 __device__ __forceinline__ uint8_t *plen(const kma_dheap *H) { return (uint8_t *)(base(H) + H->off_plen); }
 __device__ __forceinline__ char *sbdata(const kma_dheap *H, uint64_t sb) { return base(H) + H->off_data + (sb << H->sb_log2); }
 __device__ __forceinline__ uint64_t *stats(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats); }
+__device__ __forceinline__ uint64_t *pcount(const kma_dheap *H, uint32_t x) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * x); }
+__device__ __forceinline__ uint64_t psum_words(const kma_dheap *H) { return (((H->npages + 63) >> 6) + 63) >> 6; }
+__device__ __forceinline__ uint64_t *psum(const kma_dheap *H, uint32_t l) { return (uint64_t *)(base(H) + H->off_psum) + l * psum_words(H); }
 
 __device__ __forceinline__ void bump_stat(const kma_dheap *H, int k)
 {
@@ -275,25 +286,31 @@ __device__ __forceinline__ uint32_t wg_ctx_of(const kma_dheap *H, uint32_t x)
 	return x * per + (bid / KMA_NXCD) % per;
 }
 
-/* ---- superblock pool: sharded Treiber stacks, 64-bit {tag:32, top:32} ---- */
+/* ---- superblock pool: sharded Treiber stacks, 64-bit {tag:32, top:32} ----
+ * XCD x owns shards [x*S, x*S + S), S = npool/8: the first D = S/2 are its
+ * dirty shards (only x pops them), the other S - D are clean. */
 __device__ __forceinline__ uint32_t shards_per_xcd(const kma_dheap *H) { return H->npool / KMA_NXCD; }
-__device__ __forceinline__ uint32_t dirty_shard(const kma_dheap *H, uint32_t x) { return x * shards_per_xcd(H); }
-__device__ __forceinline__ uint32_t clean_shard(const kma_dheap *H, uint32_t x, uint32_t ctx)
+__device__ __forceinline__ uint32_t dirty_per_xcd(const kma_dheap *H) { return shards_per_xcd(H) / 2u; }
+__device__ __forceinline__ uint32_t dirty_shard(const kma_dheap *H, uint32_t x, uint32_t k)
 {
-	uint32_t S = shards_per_xcd(H);
-	return x * S + 1u + ctx % (S - 1u);
+	return x * shards_per_xcd(H) + k % dirty_per_xcd(H);
+}
+__device__ __forceinline__ uint32_t clean_shard(const kma_dheap *H, uint32_t x, uint32_t k)
+{
+	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
+	return x * S + D + k % (S - D);
 }
 
+/* Push never gives up: a failed CAS means another push or pop succeeded,
+ * so the pool as a whole progresses; back-off spreads the retries. */
 __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb)
 {
 	uint64_t *hp = pool(H, shard);
+	/* the XCD count is raised before the superblock becomes poppable */
+	fadd(pcount(H, shard / shards_per_xcd(H)), (uint64_t)1);
 	uint64_t h = ld(hp);
-	waiter wt;
+	uint32_t bo = 1;
 	for (;;) {
-		if (wt.expired()) {
-			bump_stat(H, KMA_STAT_WAIT_POOL);
-			return;
-		}
 		/* the link must be performed before the head CAS publishes it */
 		xchg(pnext(H, sb), (uint32_t)h);
 		order_prior();
@@ -302,6 +319,9 @@ __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb
 		if (seen == h)
 			break;
 		h = seen;
+		for (uint32_t i = 0; i < bo; i++)
+			__builtin_amdgcn_s_sleep(1);
+		bo = bo < 16u ? 2u * bo : 16u;
 	}
 	fadd(hp + 1, (uint64_t)1);
 }
@@ -311,47 +331,70 @@ __device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32
 	uint64_t *hp = pool(H, shard);
 	uint64_t h = ld(hp);
 	waiter wt;
+	uint32_t bo = 1;
 	for (;;) {
 		if (wt.expired()) {
 			bump_stat(H, KMA_STAT_WAIT_POOL);
 			return false;
 		}
 		uint32_t top = (uint32_t)h;
-		if (top == 0) {
-			/* confirm emptiness with an RMW read of the head */
-			h = fadd(hp, (uint64_t)0);
-			if ((uint32_t)h == 0)
-				return false;
-			continue;
-		}
-		uint32_t nx = fadd(pnext(H, top - 1u), 0u);
+		if (top == 0)
+			return false;
+		uint32_t nx = ld(pnext(H, top - 1u));
 		uint64_t nh = (((h >> 32) + 1ull) << 32) | (uint64_t)nx;
 		uint64_t seen = cas(hp, h, nh);
 		if (seen == h) {
 			fsub(hp + 1, (uint64_t)1);
+			fsub(pcount(H, shard / shards_per_xcd(H)), (uint64_t)1);
 			*out = top - 1u;
 			return true;
 		}
 		h = seen;
+		for (uint32_t i = 0; i < bo; i++)
+			__builtin_amdgcn_s_sleep(1);
+		bo = bo < 16u ? 2u * bo : 16u;
 	}
 }
 
-/* Pop for XCD x: its own clean shards, then its dirty shard, then the other
- * XCDs' clean shards. */
+/* Pop from shards first, first + ((start + i) % n) for i < n: their heads
+ * are read eight at a time (one round trip per eight shards) and only the
+ * non-empty ones are popped. */
+__device__ inline bool pool_pop_range(const kma_dheap *H, uint32_t first, uint32_t n, uint32_t start, uint32_t *out)
+{
+	for (uint32_t i0 = 0; i0 < n; i0 += 8) {
+		uint64_t hv[8];
+#pragma unroll
+		for (uint32_t k = 0; k < 8; k++)
+			hv[k] = i0 + k < n ? ld(pool(H, first + (start + i0 + k) % n)) : 0ull;
+#pragma unroll
+		for (uint32_t k = 0; k < 8; k++)
+			if ((uint32_t)hv[k] != 0 && pool_pop_shard(H, first + (start + i0 + k) % n, out))
+				return true;
+	}
+	return false;
+}
+
+/* Pop for XCD x: its own clean shards, then its own dirty shards, then the
+ * other XCDs' clean shards.  XCDs whose count reads zero are skipped (the
+ * eight counts are read in one round trip), so an exhausted pool fails
+ * fast. */
 __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t *out)
 {
-	uint32_t S = shards_per_xcd(H);
-	uint32_t c0 = clean_shard(H, x, ctx) - x * S - 1u;
-	for (uint32_t i = 0; i < S - 1u; i++)
-		if (pool_pop_shard(H, x * S + 1u + (c0 + i) % (S - 1u), out))
+	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
+	uint64_t cnt[KMA_NXCD];
+#pragma unroll
+	for (uint32_t y = 0; y < KMA_NXCD; y++)
+		cnt[y] = ld(pcount(H, y));
+	if (cnt[x] != 0) {
+		if (pool_pop_range(H, x * S + D, S - D, ctx, out))
 			return true;
-	if (pool_pop_shard(H, dirty_shard(H, x), out))
-		return true;
+		if (pool_pop_range(H, x * S, D, ctx, out))
+			return true;
+	}
 	for (uint32_t j = 1; j < KMA_NXCD; j++) {
 		uint32_t y = (x + j) & (KMA_NXCD - 1u);
-		for (uint32_t i = 0; i < S - 1u; i++)
-			if (pool_pop_shard(H, y * S + 1u + (c0 + i) % (S - 1u), out))
-				return true;
+		if (cnt[y] != 0 && pool_pop_range(H, y * S + D, S - D, ctx, out))
+			return true;
 	}
 	return false;
 }
@@ -370,7 +413,7 @@ __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, 
 				release_l2();
 				pool_push(H, clean_shard(H, x, ctx), sb);
 			} else {
-				pool_push(H, dirty_shard(H, home), sb);
+				pool_push(H, dirty_shard(H, home, ctx), sb);
 			}
 			return;
 		}
@@ -504,16 +547,69 @@ __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint3
 	return nullptr;
 }
 
-/* ---- large requests: page runs ---- */
-__device__ __forceinline__ uint64_t aligned_free_runs(uint64_t w, uint32_t kp)
+/* ---- large requests: page runs ----
+ * A request of k pages (1..64) takes k pages at the start of a free aligned
+ * window of 2^l >= k pages inside one 64-page bitmap word (level l).  For
+ * each level, psum(H, l) holds one bit per bitmap word: set when that word
+ * has no free window of that level.  The bits are hints that are exact
+ * whenever no call is in flight:
+ *   - a caller that saw a word without a window sets the bit, waits for the
+ *     set to complete, re-reads the word with an RMW and clears the bit
+ *     again if a window has appeared;
+ *   - a free (or the rollback of a lost claim) clears the bit of every level
+ *     that gained a window, after its own bitmap update completed.
+ * Whichever of a racing set and clear comes last, one of the two sides sees
+ * the other's bitmap update, so a window is never hidden once both finish.
+ * An allocation reads the summary words (coalesced across the wavefront)
+ * and visits only words that may hold a window, so a full page region fails
+ * after psum_words() reads. */
+__device__ __forceinline__ uint64_t page_valid(const kma_dheap *H, uint64_t wi)
 {
-	uint64_t z = ~w;
-	for (uint32_t s = 1; s < kp; s <<= 1)
-		z &= z >> s;
+	uint64_t n = H->npages - (wi << 6);
+	return n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+}
+
+/* Bit j set: pages j .. j + 2^l - 1 are free and valid, j a multiple of 2^l. */
+__device__ __forceinline__ uint64_t page_windows(uint64_t w, uint32_t l, uint64_t valid)
+{
+	uint64_t z = ~w & valid;
+	for (uint32_t sh = 1; sh < (1u << l); sh <<= 1)
+		z &= z >> sh;
 	uint64_t pat = 0;
-	for (uint32_t i = 0; i < 64; i += kp)
+	for (uint32_t i = 0; i < 64; i += 1u << l)
 		pat |= 1ull << i;
 	return z & pat;
+}
+
+__device__ __forceinline__ void psum_mark_full(const kma_dheap *H, uint32_t l, uint64_t wi)
+{
+	uint64_t *sp = psum(H, l) + (wi >> 6);
+	uint64_t bit = 1ull << (wi & 63);
+	if (f_or(sp, bit) & bit)
+		return;
+	order_prior();
+	uint64_t w = f_or(pbits(H) + wi, (uint64_t)0);
+	if (page_windows(w, l, page_valid(H, wi)))
+		f_and(sp, ~bit);
+}
+
+/* Clear the hint of every level that gained a window going from `before` to
+ * `after` (the caller's bitmap update has completed). */
+__device__ __forceinline__ void psum_gained(const kma_dheap *H, uint64_t wi, uint64_t before, uint64_t after)
+{
+	uint64_t valid = page_valid(H, wi);
+	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
+		if (!page_windows(before, l, valid) && page_windows(after, l, valid))
+			f_and(psum(H, l) + (wi >> 6), ~((uint64_t)1 << (wi & 63)));
+}
+
+/* Set the hint of every level that lost its last window. */
+__device__ __forceinline__ void psum_lost(const kma_dheap *H, uint64_t wi, uint64_t before, uint64_t after)
+{
+	uint64_t valid = page_valid(H, wi);
+	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
+		if (page_windows(before, l, valid) && !page_windows(after, l, valid))
+			psum_mark_full(H, l, wi);
 }
 
 __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t seed)
@@ -522,41 +618,66 @@ __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t s
 	uint64_t k = (size + psz - 1) >> H->page_log2;
 	if (k > 64 || H->npages == 0)
 		return nullptr;
-	uint32_t kp = 1;
-	while (kp < k)
-		kp <<= 1;
+	uint32_t l = 0;
+	while ((1ull << l) < k)
+		l++;
 	uint64_t run = (k == 64) ? ~0ull : ((1ull << k) - 1ull);
-	uint64_t nwords = (H->npages + 63) >> 6;
+	uint64_t ns = psum_words(H);
+	uint64_t *sp = psum(H, l);
 	uint64_t *pb = pbits(H);
-	uint64_t wi = ((uint64_t)seed * 0x9E3779B1u) % nwords;
-	bool fresh = false;
-	for (uint64_t probe = 0; probe < 2 * nwords;) {
-		uint64_t w = fresh ? f_or(pb + wi, (uint64_t)0) : ld(pb + wi);
-		uint64_t cand = aligned_free_runs(w, kp);
-		if (wi == nwords - 1 && (H->npages & 63)) {
-			/* runs must end inside the page range */
-			uint32_t valid = (uint32_t)(H->npages & 63);
-			uint64_t ok = 0;
-			for (uint32_t j = 0; j + k <= valid; j += kp)
-				ok |= 1ull << j;
-			cand &= ok;
+	uint32_t h = seed * 0x9E3779B1u;
+	uint64_t si = (uint64_t)(h >> 8) % ns;
+	uint32_t rot = h & 63u;
+	waiter wt;
+	for (uint64_t pass = 0; pass < ns; pass++, si = (si + 1 == ns) ? 0 : si + 1) {
+		uint64_t cand = ~ld(sp + si);
+		while (cand) {
+			if (wt.expired()) {
+				bump_stat(H, KMA_STAT_WAIT_POOL);
+				return nullptr;
+			}
+			/* start at a seed-dependent bit so callers spread out */
+			uint64_t r = (cand >> rot) | (rot ? cand << (64u - rot) : 0ull);
+			uint32_t b = ((uint32_t)__ffsll((unsigned long long)r) - 1u + rot) & 63u;
+			uint64_t wi = (si << 6) + b;
+			if (wi >= ((H->npages + 63) >> 6)) { /* no such word */
+				cand &= ~(1ull << b);
+				continue;
+			}
+			uint64_t valid = page_valid(H, wi);
+			uint64_t w = ld(pb + wi);
+			for (;;) {
+				if (wt.expired()) {
+					bump_stat(H, KMA_STAT_WAIT_POOL);
+					return nullptr;
+				}
+				uint64_t win = page_windows(w, l, valid);
+				if (!win) {
+					psum_mark_full(H, l, wi);
+					cand &= ~(1ull << b);
+					break;
+				}
+				uint32_t j = (uint32_t)__ffsll((unsigned long long)win) - 1u;
+				uint64_t m = run << j;
+				uint64_t old = f_or(pb + wi, m);
+				if ((old & m) == 0) {
+					uint64_t pi = (wi << 6) + j;
+					st(plen(H) + pi, (uint8_t)k);
+					psum_lost(H, wi, old, old | m);
+					return base(H) + H->off_pages + (pi << H->page_log2);
+				}
+				/* lost a race: give back the pages this call took */
+				uint64_t mine = m & ~old;
+				if (mine) {
+					uint64_t o2 = f_and(pb + wi, ~mine);
+					order_prior();
+					psum_gained(H, wi, o2, o2 & ~mine);
+					w = o2 & ~mine;
+				} else {
+					w = old;
+				}
+			}
 		}
-		if (!cand) {
-			wi = (wi + 1 == nwords) ? 0 : wi + 1;
-			if (++probe == nwords)
-				fresh = true; /* second sweep with RMW reads */
-			continue;
-		}
-		uint32_t j = __ffsll((unsigned long long)cand) - 1u;
-		uint64_t m = run << j;
-		uint64_t old = f_or(pb + wi, m);
-		if ((old & m) == 0) {
-			uint64_t pi = (wi << 6) + j;
-			st(plen(H) + pi, (uint8_t)k);
-			return base(H) + H->off_pages + (pi << H->page_log2);
-		}
-		if (m & ~old)
-			f_and(pb + wi, ~(m & ~old));
 	}
 	return nullptr;
 }
@@ -564,17 +685,76 @@ __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t s
 __device__ inline void page_free(const kma_dheap *H, char *p)
 {
 	uint64_t pi = (uint64_t)(p - (base(H) + H->off_pages)) >> H->page_log2;
+	if (pi >= H->npages) {
+		bump_stat(H, KMA_STAT_BAD_FREE);
+		return;
+	}
 	uint32_t k = ld(plen(H) + pi);
 	if (k == 0 || k > 64) {
 		bump_stat(H, KMA_STAT_BAD_FREE);
 		return;
 	}
+	uint64_t wi = pi >> 6;
 	uint64_t m = ((k == 64) ? ~0ull : ((1ull << k) - 1ull)) << (pi & 63);
 	/* pages are reused by any XCD: write back this XCD's L2 first */
 	release_l2();
-	uint64_t old = f_and(pbits(H) + (pi >> 6), ~m);
+	uint64_t old = f_and(pbits(H) + wi, ~m);
 	if ((old & m) != m)
 		bump_stat(H, KMA_STAT_DOUBLE_FREE);
+	order_prior();
+	psum_gained(H, wi, old, old & ~m);
+}
+
+/* ---- initialisation (every metadata word starts at zero) ----
+ * Item i of max(nsb, npool, KMA_PAGE_LEVELS * psum_words) items:
+ *   - superblocks start in the clean shards: clean shard j (of the
+ *     C = 8 * (S - D) clean shards, S - D per XCD) holds superblocks j,
+ *     j + C, j + 2C, ... (lowest on top); dirty shards start empty;
+ *   - each XCD's pooled count is the sum over its clean shards;
+ *   - a page-window summary bit starts set only for a word that does not
+ *     exist or cannot hold a window of that level (a partial last word). */
+__device__ __forceinline__ uint64_t init_shard_len(uint64_t j, uint64_t nsb, uint64_t C)
+{
+	return j < nsb ? (nsb - j + C - 1u) / C : 0u;
+}
+
+__device__ inline void init_item(const kma_dheap *H, uint64_t i)
+{
+	uint64_t nsb = H->nsb;
+	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H), K = S - D, C = K * KMA_NXCD;
+	if (i < nsb) {
+		uint64_t nx = i + C;
+		*pnext(H, i) = nx < nsb ? (uint32_t)(nx + 1u) : 0u;
+	}
+	if (i < C) {
+		uint64_t *hp = pool(H, (uint32_t)(i / K) * S + D + (uint32_t)(i % K));
+		hp[0] = i < nsb ? (uint64_t)(i + 1u) : 0u;
+		hp[1] = init_shard_len(i, nsb, C);
+	}
+	if (i < KMA_NXCD) {
+		uint64_t n = 0;
+		for (uint64_t j = i * K; j < (i + 1) * K; j++)
+			n += init_shard_len(j, nsb, C);
+		*pcount(H, (uint32_t)i) = n;
+	}
+	uint64_t ns = psum_words(H), nwords = (H->npages + 63) >> 6;
+	if (H->npages && i < KMA_PAGE_LEVELS * ns) {
+		uint32_t l = (uint32_t)(i / ns);
+		uint64_t si = i % ns, bits = 0;
+		for (uint32_t b = 0; b < 64; b++) {
+			uint64_t wi = (si << 6) + b;
+			if (wi >= nwords || !page_windows(0, l, page_valid(H, wi)))
+				bits |= (uint64_t)1 << b;
+		}
+		psum(H, l)[si] = bits;
+	}
+}
+
+__host__ __device__ inline uint64_t init_items(uint64_t nsb, uint32_t npool, uint64_t npages)
+{
+	uint64_t n = nsb > npool ? nsb : npool;
+	uint64_t nps = KMA_PAGE_LEVELS * ((((npages + 63) >> 6) + 63) >> 6);
+	return nps > n ? nps : n;
 }
 
 } /* namespace kma_nat */
@@ -645,16 +825,21 @@ __device__ inline void kma_free(kma_dheap *H, void *ptr)
 	uint64_t s = ld(state(H, sb));
 	uint32_t c1 = st_cls1(s);
 	bool live = c1 != 0;
+	/* the caller's stores to the block complete before its bit is
+	 * released; a block homed on another XCD also needs this XCD's L2
+	 * written back (one write-back and one counter update per wavefront) */
+	uint64_t rm = __ballot(live && st_home(s) != x);
+	if (rm) {
+		if (lane() == (uint32_t)__ffsll((unsigned long long)rm) - 1u)
+			fadd(stats(H) + KMA_STAT_REMOTE_FREE, (uint64_t)__popcll(rm));
+		release_l2();
+	} else {
+		order_prior();
+	}
 	if (live) {
 		uint64_t inoff = off & ((1ull << H->sb_log2) - 1ull);
 		uint32_t idx = (uint32_t)((inoff * H->class_magic[c1 - 1u]) >> 40);
 		uint32_t m = 1u << (idx & 31u);
-		if (st_home(s) != x) {
-			bump_stat(H, KMA_STAT_REMOTE_FREE);
-			release_l2();
-		} else {
-			order_prior();
-		}
 		uint32_t old = f_and(bitmap(H, sb) + (idx >> 5), ~m);
 		if (!(old & m)) {
 			bump_stat(H, KMA_STAT_DOUBLE_FREE);

@@ -259,17 +259,29 @@ __device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L
 		}
 	}
 	__syncthreads();
+	uint32_t sb = 0, wi = 0, nrem = 0;
 	if (owner) {
 		unsigned long long key = L.hkey[slot] - 1ull;
-		uint32_t sb = (uint32_t)(key / H->bm_words);
-		uint32_t wi = (uint32_t)(key % H->bm_words);
+		sb = (uint32_t)(key / H->bm_words);
+		wi = (uint32_t)(key % H->bm_words);
+		if (st_home(ld(state(H, sb))) != x)
+			nrem = L.hn[slot];
+	}
+	/* the barrier above waited for every thread's stores; a superblock
+	 * homed on another XCD also needs this XCD's L2 written back: one
+	 * write-back and one counter update per wavefront (the count is summed
+	 * bit-plane by bit-plane with ballots, which also works for a partial
+	 * last wavefront) */
+	if (__ballot(nrem != 0)) {
+		uint32_t tot = 0;
+		for (uint32_t b = 0; (1u << b) <= (uint32_t)BLOCK; b++)
+			tot += (uint32_t)__popcll(__ballot((nrem >> b) & 1u)) << b;
+		if (lane() == (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1u)
+			fadd(stats(H) + KMA_STAT_REMOTE_FREE, (uint64_t)tot);
+		release_l2();
+	}
+	if (owner) {
 		uint32_t mask = L.hmask[slot];
-		/* the barrier above waited for every thread's stores; a superblock
-		 * homed on another XCD also needs this XCD's L2 written back */
-		if (st_home(ld(state(H, sb))) != x) {
-			fadd(stats(H) + KMA_STAT_REMOTE_FREE, (uint64_t)L.hn[slot]);
-			release_l2();
-		}
 		uint32_t old = f_and(bitmap(H, sb) + wi, ~mask);
 		uint32_t n = __popc(old & mask);
 		uint32_t dbl = L.hn[slot] - n;

@@ -99,6 +99,126 @@ int kma_check_wave_claim(kma_heap *h, uint32_t ncases, const uint32_t *cls, cons
  * superblocks return to the pool (no allocation may be in flight). */
 int kma_trim(kma_heap *h);
 
+/* ------------------------------------------------------------------ */
+/* Batched request streams                                              */
+/* ------------------------------------------------------------------ */
+/* A stream is a host-generated trace of allocation batches: batch sizes are
+ * Zipf-distributed, each request is either a malloc of a log-uniform size or
+ * (with probability free_pct %) a free of a block returned by an EARLIER
+ * batch, chosen uniformly among the blocks still live.  With `drain` a last
+ * batch frees every block still live, so a run returns the heap to empty.
+ * Request j is encoded in one u32: bit 31 clear: malloc(j's low 31 bits);
+ * bit 31 set: free the block of request (low 31 bits). */
+typedef struct kma_stream_spec {
+	uint32_t nbatch;     /* batches before the drain batch */
+	uint32_t max_batch;  /* batch sizes Zipf over 1..max_batch */
+	uint32_t zipf_milli; /* Zipf exponent x 1000 */
+	uint32_t free_pct;   /* percent of requests that are frees */
+	uint32_t min_size;   /* log-uniform sizes in [min_size, max_size), powers of two */
+	uint32_t max_size;
+	uint32_t page_min;   /* sizes >= page_min are counted as large in the info */
+	uint32_t drain;      /* 1: append the drain batch */
+	uint64_t seed;
+} kma_stream_spec;
+
+typedef struct kma_stream_info {
+	uint64_t nreq;        /* requests, drain included */
+	uint64_t nbatch;      /* batches, drain included */
+	uint64_t nmalloc;
+	uint64_t nfree;
+	uint64_t max_batch;   /* largest batch drawn */
+	uint64_t peak_live;   /* most blocks live at a batch boundary */
+	uint64_t small_bytes; /* sum over mallocs < page_min of the size rounded up to a power of two */
+	uint64_t large_bytes; /* sum over mallocs >= page_min of the size rounded up to page_min multiples */
+	uint64_t peak_small_bytes; /* same sums over the blocks live at the worst batch boundary */
+	uint64_t peak_large_bytes;
+} kma_stream_info;
+
+typedef struct kma_stream kma_stream; /* opaque: host trace + device copy */
+
+/* Generate the trace on the host and upload it.  NULL on bad input. */
+kma_stream *kma_stream_create(const kma_stream_spec *spec);
+void kma_stream_destroy(kma_stream *s);
+int kma_stream_get_info(const kma_stream *s, kma_stream_info *out);
+/* Copy the trace: req[nreq], batch_off[nbatch + 1]. */
+int kma_stream_trace(const kma_stream *s, uint32_t *req, uint32_t *batch_off);
+/* After a run: per request, the heap offset a malloc returned (0: NULL;
+ * 0 for frees). */
+int kma_stream_results(const kma_stream *s, uint64_t *res);
+/* After a run with KMA_VAR_LOG: per request, the device wall-clock tick at
+ * which a malloc returned or a free was issued. */
+int kma_stream_times(const kma_stream *s, uint64_t *ticks);
+
+#define KMA_VAR_LOG 0x400u /* record per-request wall-clock ticks */
+
+typedef struct kma_stream_stats {
+	double gpu_ms;         /* HIP-event time of one replay (average over reps) */
+	double kernel_ms_avg;  /* average launch duration inside it */
+	uint64_t launches;     /* launches per replay */
+	uint64_t ops;          /* requests executed (mallocs + frees) per replay */
+	uint64_t mallocs;      /* mallocs that returned a block */
+	uint64_t nulls;        /* mallocs that returned NULL */
+	uint64_t frees;        /* frees of a block */
+	uint64_t free_nulls;   /* frees whose malloc had returned NULL (no-ops) */
+	uint64_t bad;          /* blocks whose first word lost the pattern written at malloc */
+	uint64_t wait_timeouts;/* frees that gave up waiting for their malloc's result */
+	uint64_t large;        /* mallocs above the largest class (pages) */
+	uint64_t large_nulls;
+	uint64_t double_frees, bad_frees, pool_empty, waits[4], remote_frees; /* heap counters, per replay */
+	double tick_ns;        /* wall-clock tick length for kma_stream_times */
+	double phase_ns[3];    /* KMA_VAR_PHASES: malloc / free-target wait / free time per
+				  wavefront slice (plain: per wavefront launch) */
+} kma_stream_stats;
+
+/* Run the stream on a native heap.
+ * KMA_FORM_PLAIN:   one launch per batch, per-lane kma_malloc / kma_free.
+ * KMA_FORM_BATCHED: ONE graph-captured persistent launch of nwg work-groups
+ *   (256 threads) that claim chunks of 256 * per_thread consecutive requests
+ *   and run each 256-request slice with kma_wg_malloc then kma_wg_free; a
+ *   free waits for the result of its malloc (always an earlier request).
+ * warmup untimed replays, then reps timed replays; the heap must be empty
+ * before each replay (true after a draining replay). */
+int kma_stream_run(kma_heap *h, kma_stream *s, uint32_t form, uint32_t nwg, uint32_t per_thread,
+		   uint32_t warmup, uint32_t reps, uint32_t variant, kma_stream_stats *out);
+
+/* Heap configuration for a stream: `heap_bytes` split between superblocks
+ * and pages in proportion to the stream's small / large demand
+ * (kma_stream_info), superblocks of 2^sb_log2 bytes, class table `table`:
+ *   KMA_TABLE_POW2: powers of two from min_size up to min(SB, page_min);
+ *   KMA_TABLE_HALF: the same plus the midpoints 1.5 x 2^k.
+ * Requests above the largest class take 2^page_log2-byte pages. */
+#define KMA_TABLE_POW2 0u
+#define KMA_TABLE_HALF 1u
+int kma_stream_heap_config(const kma_stream *s, uint64_t heap_bytes, uint32_t sb_log2, uint32_t table,
+			   uint32_t page_log2, kma_config *out);
+
+#define KMA_TUNE_MAX 64
+typedef struct kma_tune_trial {
+	uint32_t sb_log2, table, nwg, per_thread;
+	double ops_per_s;     /* requests per second */
+	double ms;            /* per replay */
+	uint64_t nulls;
+	uint64_t violations;  /* bad + wait_timeouts + double/bad frees + watchdog expiries */
+} kma_tune_trial;
+
+typedef struct kma_tune_result {
+	kma_config cfg;       /* chosen heap configuration */
+	uint32_t nwg, per_thread; /* chosen persistent launch shape */
+	uint32_t table;
+	uint32_t ntrials;
+	double ops_per_s;
+	uint64_t nulls;
+	kma_tune_trial trials[KMA_TUNE_MAX];
+} kma_tune_result;
+
+/* Autotuner for the persistent form: sweeps superblock size (16, 32,
+ * 64 KiB), class table (KMA_TABLE_*), work-groups (512, 1024) and requests
+ * per thread per chunk (1, 2, 4), `reps` timed replays each, and picks the
+ * highest throughput among trials with no violation and at most `null_limit`
+ * NULL returns (no limit when 0).  Every trial heap is `heap_bytes`. */
+int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, uint64_t null_limit,
+			uint32_t reps, kma_tune_result *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,7 +20,7 @@ MAX_CLASSES = 24
 WL_FIXED, WL_LOGUNI, WL_REF_TEST, WL_LOWVAR, WL_HIGHVAR = range(5)
 FORM_PLAIN, FORM_BATCHED, FORM_FUSED = 0, 1, 2
 VAR_LANE_MALLOC, VAR_LANE_FREE, VAR_LANE_CLAIM = 0x1, 0x2, 0x4
-VAR_LATENCY, VAR_PHASES = 0x100, 0x200
+VAR_LATENCY, VAR_PHASES, VAR_LOG = 0x100, 0x200, 0x400
 
 # Exported symbols, as declared in include/*.h (checked by the CPU tests).
 EXPORTS = [
@@ -28,7 +28,12 @@ EXPORTS = [
     "kma_heap_bytes", "kma_get_config", "kma_snapshot", "kma_reset",
     "kma_last_error", "kma_device_ok", "kma_run_workload", "kma_trace_serial",
     "kma_trim", "kma_atomic_peak", "kma_check_wave_claim",
+    "kma_stream_create", "kma_stream_destroy", "kma_stream_get_info",
+    "kma_stream_trace", "kma_stream_results", "kma_stream_times", "kma_stream_run",
+    "kma_stream_heap_config", "kma_stream_autotune",
 ]
+TABLE_POW2, TABLE_HALF = 0, 1
+TUNE_MAX = 64
 
 
 class Config(ctypes.Structure):
@@ -88,6 +93,71 @@ class RunStats(ctypes.Structure):
         return d
 
 
+class StreamSpec(ctypes.Structure):
+    _fields_ = [
+        ("nbatch", ctypes.c_uint32),
+        ("max_batch", ctypes.c_uint32),
+        ("zipf_milli", ctypes.c_uint32),
+        ("free_pct", ctypes.c_uint32),
+        ("min_size", ctypes.c_uint32),
+        ("max_size", ctypes.c_uint32),
+        ("page_min", ctypes.c_uint32),
+        ("drain", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class StreamInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "nreq", "nbatch", "nmalloc", "nfree", "max_batch", "peak_live", "small_bytes",
+        "large_bytes", "peak_small_bytes", "peak_large_bytes")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StreamStats(ctypes.Structure):
+    _fields_ = [
+        ("gpu_ms", ctypes.c_double),
+        ("kernel_ms_avg", ctypes.c_double),
+    ] + [(k, ctypes.c_uint64) for k in (
+        "launches", "ops", "mallocs", "nulls", "frees", "free_nulls", "bad", "wait_timeouts",
+        "large", "large_nulls", "double_frees", "bad_frees", "pool_empty")] + [
+        ("waits", ctypes.c_uint64 * 4),
+        ("remote_frees", ctypes.c_uint64),
+        ("tick_ns", ctypes.c_double),
+        ("phase_ns", ctypes.c_double * 3),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["waits"] = list(self.waits)
+        d["phase_ns"] = list(self.phase_ns)
+        return d
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [
+        ("sb_log2", ctypes.c_uint32), ("table", ctypes.c_uint32), ("nwg", ctypes.c_uint32),
+        ("per_thread", ctypes.c_uint32), ("ops_per_s", ctypes.c_double), ("ms", ctypes.c_double),
+        ("nulls", ctypes.c_uint64), ("violations", ctypes.c_uint64),
+    ]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("cfg", Config), ("nwg", ctypes.c_uint32), ("per_thread", ctypes.c_uint32),
+        ("table", ctypes.c_uint32), ("ntrials", ctypes.c_uint32), ("ops_per_s", ctypes.c_double),
+        ("nulls", ctypes.c_uint64), ("trials", TuneTrial * TUNE_MAX),
+    ]
+
+
+def config_dict(c):
+    return {"layout": c.layout, "sb_log2": c.sb_log2, "sblocks": c.sblocks, "nctx": c.nctx,
+            "npool": c.npool, "classes": list(c.class_size[:c.nclasses]), "page_log2": c.page_log2,
+            "npages": c.npages}
+
+
 class KmaError(RuntimeError):
     pass
 
@@ -125,6 +195,17 @@ def lib():
         L.kma_check_wave_claim.argtypes = [vp, u32, vp, vp, vp, vp, vp, vp, vp]
         L.kma_atomic_peak.argtypes = [u64, u64, u32, ctypes.c_int, u32, u32,
                                       P(ctypes.c_double), P(ctypes.c_double)]
+        L.kma_stream_create.argtypes = [P(StreamSpec)]
+        L.kma_stream_create.restype = vp
+        L.kma_stream_destroy.argtypes = [vp]
+        L.kma_stream_destroy.restype = None
+        L.kma_stream_get_info.argtypes = [vp, P(StreamInfo)]
+        L.kma_stream_trace.argtypes = [vp, vp, vp]
+        L.kma_stream_results.argtypes = [vp, vp]
+        L.kma_stream_times.argtypes = [vp, vp]
+        L.kma_stream_run.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, P(StreamStats)]
+        L.kma_stream_heap_config.argtypes = [vp, u64, u32, u32, u32, P(Config)]
+        L.kma_stream_autotune.argtypes = [vp, u64, u32, u64, u32, P(TuneResult)]
         _lib = L
     return _lib
 
@@ -159,6 +240,16 @@ class Heap:
         if not self._h:
             raise KmaError("kma_create_ex: " + last_error())
         self.layout = layout
+
+    @classmethod
+    def from_config(cls, c):
+        """kma_create_ex on a Config structure."""
+        self = cls.__new__(cls)
+        self._h = lib().kma_create_ex(ctypes.byref(c))
+        if not self._h:
+            raise KmaError("kma_create_ex: " + last_error())
+        self.layout = c.layout
+        return self
 
     @classmethod
     def create(cls, sblocks):
@@ -251,3 +342,74 @@ def atomic_peak(nbytes, nthreads, iters, pattern, warmup=2, reps=5):
     _check(lib().kma_atomic_peak(nbytes, nthreads, iters, pattern, warmup, reps, ctypes.byref(ops),
                                  ctypes.byref(ms)), "kma_atomic_peak")
     return ops.value, ms.value
+
+
+class Stream:
+    """A batched request stream (kma_stream_create): Zipf batch sizes,
+    log-uniform request sizes, frees of earlier batches' blocks, and an
+    optional drain batch that frees everything still live."""
+
+    def __init__(self, nbatch, max_batch, zipf=1.2, free_pct=40, min_size=16, max_size=65536,
+                 page_min=16384, drain=True, seed=3):
+        sp = StreamSpec(nbatch, max_batch, int(round(zipf * 1000)), free_pct, min_size, max_size,
+                        page_min, 1 if drain else 0, seed)
+        self._s = lib().kma_stream_create(ctypes.byref(sp))
+        if not self._s:
+            raise KmaError("kma_stream_create: " + last_error())
+        inf = StreamInfo()
+        _check(lib().kma_stream_get_info(self._s, ctypes.byref(inf)), "kma_stream_get_info")
+        self.info = inf.as_dict()
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib().kma_stream_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def trace(self):
+        """(req u32[nreq], batch_off u32[nbatch + 1])"""
+        req = np.zeros(self.info["nreq"], dtype=np.uint32)
+        boff = np.zeros(self.info["nbatch"] + 1, dtype=np.uint32)
+        _check(lib().kma_stream_trace(self._s, req.ctypes.data, boff.ctypes.data), "kma_stream_trace")
+        return req, boff
+
+    def results(self):
+        res = np.zeros(self.info["nreq"], dtype=np.uint64)
+        _check(lib().kma_stream_results(self._s, res.ctypes.data), "kma_stream_results")
+        return res
+
+    def times(self):
+        t = np.zeros(self.info["nreq"], dtype=np.uint64)
+        _check(lib().kma_stream_times(self._s, t.ctypes.data), "kma_stream_times")
+        return t
+
+    def run(self, heap, form=FORM_BATCHED, nwg=1024, per_thread=4, warmup=0, reps=1, variant=0):
+        st = StreamStats()
+        _check(lib().kma_stream_run(heap._h, self._s, form, nwg, per_thread, warmup, reps, variant,
+                                    ctypes.byref(st)), "kma_stream_run")
+        return st.as_dict()
+
+    def heap_config(self, heap_bytes, sb_log2=14, table=TABLE_POW2, page_log2=14):
+        """Native heap Config for this stream (kma_stream_heap_config)."""
+        c = Config()
+        _check(lib().kma_stream_heap_config(self._s, heap_bytes, sb_log2, table, page_log2,
+                                            ctypes.byref(c)), "kma_stream_heap_config")
+        return c
+
+    def autotune(self, heap_bytes, page_log2=14, null_limit=0, reps=2):
+        """Sweep superblock size, class table and launch shape for the
+        persistent form (kma_stream_autotune)."""
+        r = TuneResult()
+        _check(lib().kma_stream_autotune(self._s, heap_bytes, page_log2, null_limit, reps,
+                                         ctypes.byref(r)), "kma_stream_autotune")
+        trials = [{k: getattr(t, k) for k, _ in TuneTrial._fields_} for t in r.trials[:r.ntrials]]
+        return {"cfg": r.cfg, "config": config_dict(r.cfg), "nwg": r.nwg, "per_thread": r.per_thread,
+                "table": r.table, "ops_per_s": r.ops_per_s, "nulls": r.nulls, "trials": trials}

@@ -18,20 +18,11 @@
 #include "kma_layout.h"
 #include "kma_work.h"
 
-#define NAT_NSTATS 8u
-#define NAT_STAT_DOUBLE_FREE 0
-#define NAT_STAT_BAD_FREE 1
-
-struct kma_heap {
-	kma_config cfg;
-	char *dev;        /* device buffer == header address */
-	uint64_t bytes;   /* bytes allocated */
-	nat_hdr nh;       /* native header (host copy) */
-};
+#include "kma_internal.h"
 
 static __thread char g_err[256];
 
-static void set_err(const char *fmt, ...)
+void kma_i_set_err(const char *fmt, ...)
 {
 	va_list ap;
 	va_start(ap, fmt);
@@ -43,25 +34,6 @@ const char *kma_last_error(void)
 {
 	return g_err;
 }
-
-#define HIPCHK(call, ret)                                                            \
-	do {                                                                         \
-		hipError_t e_ = (call);                                              \
-		if (e_ != hipSuccess) {                                              \
-			set_err("%s failed: %s", #call, hipGetErrorString(e_));      \
-			return ret;                                                  \
-		}                                                                    \
-	} while (0)
-
-#define LCHK(call, ret)                                                              \
-	do {                                                                         \
-		int r_ = (call);                                                     \
-		if (r_ != 0) {                                                       \
-			set_err("%s failed: %d (%s)", #call, r_,                     \
-				hipGetErrorString((hipError_t)(-r_)));               \
-			return ret;                                                  \
-		}                                                                    \
-	} while (0)
 
 int kma_device_ok(void)
 {
@@ -293,7 +265,7 @@ int kma_trim(kma_heap *h)
 	return 0;
 }
 
-static int read_native_stats(kma_heap *h, uint64_t *st)
+int kma_i_read_native_stats(kma_heap *h, uint64_t *st)
 {
 	if (h->cfg.layout != KMA_LAYOUT_NATIVE) {
 		memset(st, 0, NAT_NSTATS * 8);
@@ -310,7 +282,7 @@ static int cmp_u32(const void *a, const void *b)
 }
 
 /* nanoseconds per tick of the device's constant wall clock */
-static double wall_clock_ns(void)
+double kma_i_wall_clock_ns(void)
 {
 	int dev = 0, khz = 0;
 	if (hipGetDevice(&dev) != hipSuccess ||
@@ -375,7 +347,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		}
 	}
 	if (hipStreamSynchronize(s) != hipSuccess || hipMemset(d_stats, 0, sizeof(hs)) != hipSuccess ||
-	    read_native_stats(h, st0) != 0) {
+	    kma_i_read_native_stats(h, st0) != 0) {
 		set_err("warm-up failed");
 		rc = -1;
 		goto done;
@@ -433,7 +405,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	hipEventElapsedTime(&ms, e0, e1);
 	out->gpu_ms = ms;
 	out->kernel_ms_avg = out->launches ? ms / (double)out->launches : 0;
-	if (hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess || read_native_stats(h, st1) != 0) {
+	if (hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess || kma_i_read_native_stats(h, st1) != 0) {
 		set_err("stats download failed");
 		rc = -1;
 		goto done;
@@ -448,7 +420,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	for (int k = 0; k < 4; k++)
 		out->waits[k] = st1[3 + k] - st0[3 + k];
 	out->remote_frees = st1[7] - st0[7];
-	double tick = wall_clock_ns();
+	double tick = kma_i_wall_clock_ns();
 	if (hs[7])
 		for (int k = 0; k < 3; k++)
 			out->phase_ns[k] = tick * (double)hs[4 + k] / (double)hs[7];

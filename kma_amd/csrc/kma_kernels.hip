@@ -6,10 +6,17 @@
 
 #include "kma_device.h"
 #include "kma_launch.h"
+#include "kma_layout.h"
 #include "kma_ref_device.h"
 #include "kma_wg_device.h"
 #include "kma_work.h"
 #include "kma_workloads.h"
+
+/* the host mirror of the device header must match it byte for byte */
+static_assert(sizeof(kma_dheap) == sizeof(nat_hdr), "kma_dheap / nat_hdr size");
+static_assert(offsetof(kma_dheap, off_psum) == offsetof(nat_hdr, off_psum), "off_psum");
+static_assert(offsetof(kma_dheap, class_magic) == offsetof(nat_hdr, class_magic), "class_magic");
+static_assert(KMA_STATS_BYTES == NAT_STATS_BYTES && KMA_PAGE_LEVELS == NAT_PAGE_LEVELS, "layout constants");
 
 #define KMA_CHECK(x)                                   \
 	do {                                           \
@@ -75,41 +82,25 @@ extern "C" int kmad_ref_init(void *heap, uint32_t log2, uint32_t sblocks)
 /* ------------------------------------------------------------------ */
 /* native layout: metadata init                                         */
 /* ------------------------------------------------------------------ */
-/* Every superblock starts in a clean shard: clean shard j (of the npool*7/8
- * ... (S-1 per XCD) holds superblocks j, j+C, j+2C, ... (lowest on top), with
- * C the number of clean shards; dirty shards start empty.  Every other
- * metadata word starts at zero (done by the host memset). */
-__device__ __forceinline__ uint32_t clean_index_to_shard(uint32_t j, uint32_t S)
+__global__ void k_nat_init(kma_dheap *H, uint64_t n)
 {
-	return (j / (S - 1u)) * S + 1u + j % (S - 1u);
-}
-
-__global__ void k_nat_init(kma_dheap *H, uint64_t nsb, uint32_t npool)
-{
-	using namespace kma_nat;
 	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	uint32_t S = npool / KMA_NXCD, C = npool - KMA_NXCD;
-	if (i < nsb) {
-		uint64_t nx = i + C;
-		*pnext(H, i) = nx < nsb ? (uint32_t)(nx + 1u) : 0u;
-	}
-	if (i < C) {
-		uint64_t *hp = pool(H, clean_index_to_shard((uint32_t)i, S));
-		hp[0] = i < nsb ? (uint64_t)(i + 1u) : 0u;
-		hp[1] = i < nsb ? (nsb - i + C - 1u) / C : 0u;
-	}
+	if (i < n)
+		kma_nat::init_item(H, i);
 }
 
 extern "C" int kmad_nat_init(void *heap, uint64_t nsb, uint32_t npool, uint64_t nheads, uint64_t npbwords)
 {
 	(void)nheads;
-	(void)npbwords;
 	uint64_t n = nsb > npool ? nsb : npool;
+	uint64_t nps = KMA_PAGE_LEVELS * ((npbwords + 63) / 64);
+	if (nps > n)
+		n = nps;
 	if (npool % KMA_NXCD || npool / KMA_NXCD < 2)
 		return -1;
 	uint32_t threads = 256;
 	uint64_t blocks = (n + threads - 1) / threads;
-	hipLaunchKernelGGL(k_nat_init, dim3((uint32_t)blocks), dim3(threads), 0, 0, (kma_dheap *)heap, nsb, npool);
+	hipLaunchKernelGGL(k_nat_init, dim3((uint32_t)blocks), dim3(threads), 0, 0, (kma_dheap *)heap, n);
 	KMA_CHECK(hipGetLastError());
 	KMA_CHECK(hipDeviceSynchronize());
 	return 0;
@@ -142,8 +133,9 @@ __global__ void k_clean_dirty(kma_dheap *H)
 		return;
 	for (uint32_t x = 0; x < KMA_NXCD; x++) {
 		uint32_t sb, n = 0;
-		while (pool_pop_shard(H, dirty_shard(H, x), &sb))
-			pool_push(H, clean_shard(H, x, n++), sb);
+		for (uint32_t k = 0; k < dirty_per_xcd(H); k++)
+			while (pool_pop_shard(H, dirty_shard(H, x, k), &sb))
+				pool_push(H, clean_shard(H, x, n++), sb);
 	}
 }
 

@@ -42,6 +42,21 @@ int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t *cls, cons
 int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, uint32_t iters,
 		     int pattern, void *stream);
 
+/* Batched request streams (kma_stream.hip).  ctl is a device array of
+ * KMA_CTL_WORDS u64: epoch, chunk counter and counters. */
+#define KMA_STREAM_BLOCK 256
+#define KMA_CTL_EPOCH 0
+#define KMA_CTL_NEXT 16
+#define KMA_CTL_STATS 32
+#define KMA_CTL_WORDS 64
+
+int kmad_stream_begin(uint64_t *d_ctl, void *stream);
+int kmad_stream_plain(void *heap, const uint32_t *d_req, uint64_t *d_res, uint64_t lo, uint64_t n,
+		      uint64_t *d_ctl, uint64_t *d_ts, uint32_t variant, void *stream);
+int kmad_stream_persistent(void *heap, const uint32_t *d_req, uint64_t *d_res, uint64_t nreq,
+			   uint32_t nwg, uint32_t per_thread, uint64_t *d_ctl, uint64_t *d_ts,
+			   uint32_t variant, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

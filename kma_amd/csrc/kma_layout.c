@@ -52,7 +52,7 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 		return -1;
 	}
 	/* contexts and pool shards are split evenly over the 8 XCDs; each XCD
-	 * has one dirty shard and at least one clean shard */
+	 * has at least one dirty and one clean shard */
 	if (cfg->nctx == 0) {
 		uint64_t want = cfg->sblocks / (4ull * cfg->nclasses);
 		uint32_t n = 8;
@@ -109,7 +109,7 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 	}
 	uint64_t off = align_up(sizeof(nat_hdr), 256);
 	nh->off_stats = off;
-	off += 256;
+	off += NAT_STATS_BYTES;
 	nh->off_state = off;
 	off = align_up(off + nh->nsb * 8, 256);
 	nh->off_next = off;
@@ -124,6 +124,8 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 	off = align_up(off + ((nh->npages + 63) / 64) * 8, 256);
 	nh->off_plen = off;
 	off = align_up(off + nh->npages, 256);
+	nh->off_psum = off;
+	off = align_up(off + NAT_PAGE_LEVELS * ((((nh->npages + 63) / 64) + 63) / 64) * 8, 256);
 	/* metadata bytes; data placed after allocation so it is absolutely
 	 * aligned to the superblock and page sizes */
 	nh->off_data = off;

@@ -41,11 +41,14 @@ typedef struct nat_hdr {
 	uint32_t class_cap[32];
 	uint64_t class_magic[32];
 	uint8_t class_for_log2[64];
+	uint64_t off_psum;
 } nat_hdr;
 
 #define NAT_MAGIC 0x4e414d4b31414d4bull
 #define NAT_HEADS_PER_CTX 32u
 #define NAT_POOL_STRIDE_BYTES 128u
+#define NAT_STATS_BYTES (256u + 128u * 8u) /* counters + one line per XCD */
+#define NAT_PAGE_LEVELS 7u
 
 /* Validate `in`, fill defaults into *out and the header (metadata offsets;
  * off_data holds the metadata size until kma_native_place runs).

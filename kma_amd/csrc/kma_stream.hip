@@ -1,0 +1,293 @@
+/*
+ * kma_stream.hip - gfx950 kernels that run a batched request stream
+ * (include/kma_work.h, kma_stream_*) on a native heap:
+ *   k_stream_plain      one launch per batch, per-lane kma_malloc / kma_free;
+ *   k_stream_persistent one launch for the whole stream: work-groups claim
+ *                       chunks of consecutive requests, serve the chunk's
+ *                       mallocs slice by slice (256 requests) with
+ *                       kma_wg_malloc, then its frees with kma_wg_free.
+ *
+ * Every malloc writes a pattern derived from its request index into the
+ * block's first word and then publishes (epoch << 40 | heap offset) in the
+ * result slot of its request.  A free reads its malloc's slot (waiting while
+ * the slot still holds an older epoch), checks the pattern and frees the
+ * block.  Frees only name requests of earlier batches, i.e. lower request
+ * indices, and a chunk runs all its mallocs before any of its frees, so a
+ * free only ever waits for a malloc pass that waits on nothing: the
+ * persistent kernel always drains.  Pattern and result are written with
+ * agent-scope atomic stores, and the pattern store completes before the
+ * result store is issued, so a free running on another XCD sees both.
+ */
+#include <hip/hip_runtime.h>
+
+#include "kma_device.h"
+#include "kma_launch.h"
+#include "kma_wg_device.h"
+#include "kma_work.h"
+
+#define KMA_CHECK(x)                                   \
+	do {                                           \
+		hipError_t e_ = (x);                   \
+		if (e_ != hipSuccess)                  \
+			return -(int)e_;               \
+	} while (0)
+
+namespace {
+
+constexpr int SBLOCK = KMA_STREAM_BLOCK;
+/* per-thread counters, summed per wavefront into ctl[KMA_CTL_STATS + k] */
+enum { C_OPS, C_MOK, C_NULL, C_FOK, C_FNULL, C_BAD, C_TMO, C_LARGE, C_LNULL, C_N };
+constexpr uint64_t OFF_MASK = (1ull << 40) - 1ull;
+/* a free gives up waiting for its malloc's result after 2 s of wall clock */
+constexpr uint64_t RESULT_WAIT_TICKS = 200000000ull;
+
+struct counters {
+	uint32_t v[C_N];
+};
+
+__device__ __forceinline__ uint32_t pattern(uint64_t j)
+{
+	return (uint32_t)(j * 0x9E3779B1ull) ^ 0xA5A5A5A5u;
+}
+
+/* Per-wavefront phase clock (KMA_VAR_PHASES): wall-clock ticks spent in
+ * malloc, in waiting for / checking free targets, and in free. */
+struct phases {
+	uint64_t t = 0, acc[3] = {0, 0, 0}, n = 0;
+	bool on;
+	__device__ explicit phases(bool enable) : on(enable) {}
+	__device__ __forceinline__ void start()
+	{
+		if (on)
+			t = wall_clock64();
+	}
+	__device__ __forceinline__ void mark(int k)
+	{
+		if (on) {
+			uint64_t u = wall_clock64();
+			acc[k] += u - t;
+			t = u;
+		}
+	}
+};
+
+/* all 64 lanes of the wavefront are active when this runs */
+__device__ void flush(uint64_t *ctl, const counters &c, const phases &ph)
+{
+	for (int k = 0; k < C_N; k++) {
+		uint64_t x = c.v[k];
+		for (int o = 32; o > 0; o >>= 1)
+			x += __shfl_xor(x, o);
+		if (__lane_id() == 0 && x)
+			atomicAdd((unsigned long long *)(ctl + KMA_CTL_STATS + k), (unsigned long long)x);
+	}
+	if (ph.on && __lane_id() == 0) {
+		for (int k = 0; k < 3; k++)
+			atomicAdd((unsigned long long *)(ctl + KMA_CTL_STATS + C_N + k), (unsigned long long)ph.acc[k]);
+		atomicAdd((unsigned long long *)(ctl + KMA_CTL_STATS + C_N + 3), (unsigned long long)ph.n);
+	}
+}
+
+__device__ __forceinline__ void count_malloc(const kma_dheap *H, uint32_t size, const void *p, counters &c)
+{
+	bool large = size > H->max_small;
+	/* constant indices only: a computed index would put c in scratch */
+	c.v[C_MOK] += p != nullptr;
+	c.v[C_NULL] += p == nullptr;
+	c.v[C_LARGE] += large;
+	c.v[C_LNULL] += large && !p;
+}
+
+/* The block of request t's malloc: waits for the result slot to carry this
+ * run's epoch, checks the pattern.  NULL if the malloc failed (or the wait
+ * gave up, which is counted). */
+__device__ void *take_target(const kma_dheap *H, uint64_t *res, uint32_t t, uint32_t ep, counters &c)
+{
+	using namespace kma_nat;
+	uint64_t v = ld(res + t);
+	if ((v >> 40) != ep) {
+		uint64_t t0 = wall_clock64();
+		uint32_t bo = 1;
+		for (;;) {
+			for (uint32_t i = 0; i < bo; i++)
+				__builtin_amdgcn_s_sleep(2);
+			bo = bo < 32u ? 2u * bo : 32u;
+			v = fadd(res + t, (uint64_t)0);
+			if ((v >> 40) == ep)
+				break;
+			if (wall_clock64() - t0 > RESULT_WAIT_TICKS) {
+				c.v[C_TMO]++;
+				return nullptr;
+			}
+		}
+	}
+	uint64_t off = v & OFF_MASK;
+	if (!off) {
+		c.v[C_FNULL]++;
+		return nullptr;
+	}
+	char *p = base(H) + off;
+	if (ld((uint32_t *)p) != pattern(t))
+		c.v[C_BAD]++;
+	c.v[C_FOK]++;
+	return p;
+}
+
+__device__ __forceinline__ uint64_t result_word(const kma_dheap *H, const void *p, uint32_t ep)
+{
+	return ((uint64_t)ep << 40) | (p ? (uint64_t)((const char *)p - kma_nat::base(H)) : 0ull);
+}
+
+} // namespace
+
+/* Start of a replay: next epoch (never 0, the value of a cleared slot),
+ * chunk counter and counters to zero. */
+__global__ void k_stream_begin(uint64_t *ctl)
+{
+	if (blockIdx.x != 0 || threadIdx.x != 0)
+		return;
+	uint64_t e = (ctl[KMA_CTL_EPOCH] + 1u) & 0xffffffu;
+	ctl[KMA_CTL_EPOCH] = e ? e : 1u;
+	ctl[KMA_CTL_NEXT] = 0;
+	for (int k = 0; k < 16; k++)
+		ctl[KMA_CTL_STATS + k] = 0;
+}
+
+__global__ void __launch_bounds__(SBLOCK) k_stream_plain(kma_dheap *H, const uint32_t *req, uint64_t *res, uint64_t lo,
+							 uint64_t hi, uint64_t *ctl, uint64_t *ts, uint32_t variant)
+{
+	using namespace kma_nat;
+	const uint32_t ep = (uint32_t)ctl[KMA_CTL_EPOCH];
+	const uint64_t j = lo + (uint64_t)blockIdx.x * SBLOCK + threadIdx.x;
+	const bool log = (variant & KMA_VAR_LOG) != 0;
+	counters c = {};
+	phases ph((variant & KMA_VAR_PHASES) != 0);
+	const bool in = j < hi;
+	const uint32_t r = in ? req[j] : 0u;
+	const bool isf = in && (r >> 31);
+	const bool ism = in && !isf;
+	void *p = nullptr;
+	ph.start();
+	if (ism) {
+		p = kma_malloc(H, r);
+		count_malloc(H, r, p, c);
+		if (p)
+			st((uint32_t *)p, pattern(j));
+	}
+	order_prior(); /* pattern stores complete before results publish */
+	if (ism) {
+		st(res + j, result_word(H, p, ep));
+		if (log)
+			ts[j] = wall_clock64();
+	}
+	ph.mark(0);
+	void *q = nullptr;
+	if (isf) {
+		q = take_target(H, res, r & 0x7fffffffu, ep, c);
+		if (log)
+			ts[j] = wall_clock64();
+	}
+	ph.mark(1);
+	if (isf)
+		kma_free(H, q);
+	ph.mark(2);
+	ph.n = 1;
+	c.v[C_OPS] = in;
+	flush(ctl, c, ph);
+}
+
+__global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, const uint32_t *req, uint64_t *res,
+							      uint64_t nreq, uint32_t per_thread, uint64_t *ctl,
+							      uint64_t *ts, uint32_t variant)
+{
+	using namespace kma_nat;
+	__shared__ kma_wg_lds<SBLOCK> L;
+	__shared__ uint64_t s_base;
+	const uint32_t ep = (uint32_t)ctl[KMA_CTL_EPOCH];
+	const uint32_t tid = threadIdx.x;
+	const bool log = (variant & KMA_VAR_LOG) != 0;
+	const uint64_t chunk = (uint64_t)SBLOCK * per_thread;
+	counters c = {};
+	phases ph((variant & KMA_VAR_PHASES) != 0);
+	for (;;) {
+		if (tid == 0)
+			s_base = fadd(ctl + KMA_CTL_NEXT, chunk);
+		__syncthreads();
+		const uint64_t b0 = s_base;
+		__syncthreads();
+		if (b0 >= nreq)
+			break;
+		/* every malloc of the chunk first, then its frees: a malloc never
+		 * waits behind a free, so a free waits at most for other chunks'
+		 * malloc passes and waits cannot chain */
+		const uint32_t nsl = (uint32_t)min((uint64_t)per_thread, (nreq - b0 + SBLOCK - 1) / SBLOCK);
+		ph.start();
+		for (uint32_t m = 0; m < nsl; m++) {
+			const uint64_t j = b0 + (uint64_t)m * SBLOCK + tid;
+			const bool in = j < nreq;
+			const uint32_t r = in ? req[j] : 0u;
+			const uint32_t size = (in && !(r >> 31)) ? r : 0u;
+			void *p = kma_wg_malloc<SBLOCK>(H, size, L);
+			if (size) {
+				count_malloc(H, size, p, c);
+				if (p)
+					st((uint32_t *)p, pattern(j));
+			}
+			order_prior();
+			if (size) {
+				st(res + j, result_word(H, p, ep));
+				if (log)
+					ts[j] = wall_clock64();
+			}
+			c.v[C_OPS] += in;
+		}
+		ph.mark(0);
+		for (uint32_t m = 0; m < nsl; m++) {
+			const uint64_t j = b0 + (uint64_t)m * SBLOCK + tid;
+			const bool in = j < nreq;
+			const uint32_t r = in ? req[j] : 0u;
+			void *q = nullptr;
+			if (in && (r >> 31)) {
+				q = take_target(H, res, r & 0x7fffffffu, ep, c);
+				if (log)
+					ts[j] = wall_clock64();
+			}
+			ph.mark(1);
+			kma_wg_free<SBLOCK>(H, q, L);
+			ph.mark(2);
+		}
+		ph.n += nsl;
+	}
+	flush(ctl, c, ph);
+}
+
+extern "C" int kmad_stream_begin(uint64_t *ctl, void *stream)
+{
+	hipLaunchKernelGGL(k_stream_begin, dim3(1), dim3(64), 0, (hipStream_t)stream, ctl);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
+extern "C" int kmad_stream_plain(void *heap, const uint32_t *req, uint64_t *res, uint64_t lo, uint64_t n,
+				 uint64_t *ctl, uint64_t *ts, uint32_t variant, void *stream)
+{
+	if (n == 0)
+		return 0;
+	uint64_t blocks = (n + SBLOCK - 1) / SBLOCK;
+	hipLaunchKernelGGL(k_stream_plain, dim3((uint32_t)blocks), dim3(SBLOCK), 0, (hipStream_t)stream,
+			   (kma_dheap *)heap, req, res, lo, lo + n, ctl, ts, variant);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
+extern "C" int kmad_stream_persistent(void *heap, const uint32_t *req, uint64_t *res, uint64_t nreq, uint32_t nwg,
+				      uint32_t per_thread, uint64_t *ctl, uint64_t *ts, uint32_t variant,
+				      void *stream)
+{
+	if (nwg == 0 || per_thread == 0)
+		return -1;
+	hipLaunchKernelGGL(k_stream_persistent, dim3(nwg), dim3(SBLOCK), 0, (hipStream_t)stream, (kma_dheap *)heap,
+			   req, res, nreq, per_thread, ctl, ts, variant);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}

@@ -25,18 +25,10 @@ dim3 g_sim_block_dim, g_sim_grid_dim;
 
 static void nat_init(kma_dheap *H)
 {
-	/* same distribution as k_nat_init: every superblock in a clean shard */
-	using namespace kma_nat;
-	uint32_t S = H->npool / KMA_NXCD, C = H->npool - KMA_NXCD;
-	for (uint64_t i = 0; i < H->nsb; i++) {
-		uint64_t nx = i + C;
-		*pnext(H, i) = nx < H->nsb ? (uint32_t)(nx + 1u) : 0u;
-	}
-	for (uint32_t j = 0; j < C; j++) {
-		uint64_t *hp = pool(H, (j / (S - 1u)) * S + 1u + j % (S - 1u));
-		hp[0] = j < H->nsb ? (uint64_t)(j + 1u) : 0u;
-		hp[1] = j < H->nsb ? (H->nsb - j + C - 1u) / C : 0u;
-	}
+	/* the device init, one item at a time */
+	uint64_t n = kma_nat::init_items(H->nsb, H->npool, H->npages);
+	for (uint64_t i = 0; i < n; i++)
+		kma_nat::init_item(H, i);
 }
 
 static void trim(kma_dheap *H)
@@ -54,8 +46,9 @@ static void trim(kma_dheap *H)
 		}
 	for (uint32_t x = 0; x < KMA_NXCD; x++) {
 		uint32_t sb, n = 0;
-		while (pool_pop_shard(H, dirty_shard(H, x), &sb))
-			pool_push(H, clean_shard(H, x, n++), sb);
+		for (uint32_t k = 0; k < dirty_per_xcd(H); k++)
+			while (pool_pop_shard(H, dirty_shard(H, x, k), &sb))
+				pool_push(H, clean_shard(H, x, n++), sb);
 	}
 }
 
@@ -95,6 +88,18 @@ static int canonical_problems(kma_dheap *H)
 	}
 	if (n != H->nsb)
 		probs++;
+	/* each XCD's pooled count equals what its shards hold */
+	uint32_t S = H->npool / KMA_NXCD;
+	for (uint32_t x = 0; x < KMA_NXCD; x++) {
+		uint64_t held = 0;
+		for (uint32_t s = x * S; s < (x + 1) * S; s++)
+			held += pool(H, s)[1];
+		if (*pcount(H, x) != held) {
+			fprintf(stderr, "xcd %u count %llu holds %llu\n", x, (unsigned long long)*pcount(H, x),
+				(unsigned long long)held);
+			probs++;
+		}
+	}
 	return probs;
 }
 

@@ -11,9 +11,11 @@ HDR_DTYPE = np.dtype([
     ("off_pages", "<u8"), ("off_stats", "<u8"),
     ("class_size", "<u4", 32), ("class_cap", "<u4", 32), ("class_magic", "<u8", 32),
     ("class_for_log2", "u1", 64),
+    ("off_psum", "<u8"),
 ])
 
 MAGIC = 0x4E414D4B31414D4B
+PAGE_LEVELS = 7
 HEADS_PER_CTX = 32
 POOL_STRIDE = 16  # u64 words
 
@@ -47,8 +49,12 @@ class NatHeap:
             off = int(off)
             return np.frombuffer(img[off: off + n * np.dtype(dt).itemsize].tobytes(), dtype=dt)
 
+        nwords = (self.npages + 63) // 64
+        ns = (nwords + 63) // 64
         return {
             "stats": arr(h["off_stats"], 8, "<u8"),
+            "pcount": arr(int(h["off_stats"]) + 256, 8 * 16, "<u8")[::16],
+            "psum": arr(h["off_psum"], PAGE_LEVELS * ns, "<u8").reshape(PAGE_LEVELS, ns),
             "state": arr(h["off_state"], self.nsb, "<u8"),
             "next": arr(h["off_next"], self.nsb, "<u4"),
             "bitmap": arr(h["off_bitmap"], self.nsb * self.bm_words, "<u4"),
@@ -95,7 +101,27 @@ class NatHeap:
         if sorted(seen) != list(range(self.nsb)):
             probs.append("pool holds %d superblocks (%d distinct), expected %d"
                          % (len(seen), len(set(seen)), self.nsb))
+        per = self.npool // 8
+        for x in range(8):
+            held = int(m["pool"][x * per:(x + 1) * per, 1].sum())
+            if int(m["pcount"][x]) != held:
+                probs.append("xcd %d pooled count %d, shards hold %d" % (x, int(m["pcount"][x]), held))
+        if self.npages and not np.array_equal(m["psum"], self.psum_initial()):
+            probs.append("page-window summaries differ from the empty-heap image")
         return probs
+
+    def psum_initial(self):
+        """Page-window summary bits of an empty page region: set only for
+        words that do not exist or cannot hold a window of the level."""
+        nwords = (self.npages + 63) // 64
+        ns = (nwords + 63) // 64
+        out = np.zeros((PAGE_LEVELS, ns), dtype=np.uint64)
+        for lv in range(PAGE_LEVELS):
+            for wi in range(ns * 64):
+                valid = min(64, max(0, self.npages - 64 * wi))
+                if valid < (1 << lv):  # no aligned window of 2^lv valid pages
+                    out[lv, wi // 64] |= np.uint64(1 << (wi % 64))
+        return out
 
 
 def check_blocks(nh, offs, sizes):
