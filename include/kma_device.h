@@ -106,8 +106,8 @@ typedef struct kma_dheap {
 #define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
 #define KMA_NSTATS 8
 /* The stats block is 256 B of counters followed by one 128-B line per XCD
- * holding that XCD's pooled-superblock count (an upper bound: it is raised
- * before a push publishes and lowered after a pop succeeds). */
+ * holding that XCD's shard mask: bit k set when shard x*S + k may hold a
+ * superblock (see pool_push / pool_pop_shard). */
 #define KMA_STATS_BYTES (256u + 128u * 8u)
 /* Page runs of k pages use an aligned window of 2^l >= k pages inside one
  * 64-page bitmap word; levels l = 0..6. */
@@ -246,7 +246,7 @@ __device__ __forceinline__ uint64_t *pbits(const kma_dheap *H) { return (uint64_
 __device__ __forceinline__ uint8_t *plen(const kma_dheap *H) { return (uint8_t *)(base(H) + H->off_plen); }
 __device__ __forceinline__ char *sbdata(const kma_dheap *H, uint64_t sb) { return base(H) + H->off_data + (sb << H->sb_log2); }
 __device__ __forceinline__ uint64_t *stats(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats); }
-__device__ __forceinline__ uint64_t *pcount(const kma_dheap *H, uint32_t x) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * x); }
+__device__ __forceinline__ uint64_t *pmask(const kma_dheap *H, uint32_t x) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * x); }
 __device__ __forceinline__ uint64_t psum_words(const kma_dheap *H) { return (((H->npages + 63) >> 6) + 63) >> 6; }
 __device__ __forceinline__ uint64_t *psum(const kma_dheap *H, uint32_t l) { return (uint64_t *)(base(H) + H->off_psum) + l * psum_words(H); }
 
@@ -287,8 +287,16 @@ __device__ __forceinline__ uint32_t wg_ctx_of(const kma_dheap *H, uint32_t x)
 }
 
 /* ---- superblock pool: sharded Treiber stacks, 64-bit {tag:32, top:32} ----
- * XCD x owns shards [x*S, x*S + S), S = npool/8: the first D = S/2 are its
- * dirty shards (only x pops them), the other S - D are clean. */
+ * XCD x owns shards [x*S, x*S + S), S = npool/8 <= 64: the first D = S/2
+ * are its dirty shards (only x pops them), the other S - D are clean.
+ * pmask(x) bit k is a hint that shard x*S + k is non-empty, updated only
+ * when a shard goes empty <-> non-empty:
+ *   - a push that found the shard empty sets the bit after its CAS;
+ *   - a pop that left the shard empty (or found it empty) clears the bit,
+ *     waits for that to complete and re-reads the head, setting the bit
+ *     again if a push got in.
+ * So a non-empty shard's bit is clear only while a push onto it is in
+ * flight; a set bit on an empty shard costs one wasted probe. */
 __device__ __forceinline__ uint32_t shards_per_xcd(const kma_dheap *H) { return H->npool / KMA_NXCD; }
 __device__ __forceinline__ uint32_t dirty_per_xcd(const kma_dheap *H) { return shards_per_xcd(H) / 2u; }
 __device__ __forceinline__ uint32_t dirty_shard(const kma_dheap *H, uint32_t x, uint32_t k)
@@ -300,14 +308,27 @@ __device__ __forceinline__ uint32_t clean_shard(const kma_dheap *H, uint32_t x, 
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
 	return x * S + D + k % (S - D);
 }
+__device__ __forceinline__ uint64_t shard_bit(const kma_dheap *H, uint32_t shard)
+{
+	return (uint64_t)1 << (shard % shards_per_xcd(H));
+}
+
+/* The shard went (or was found) empty: clear its hint unless a push got in. */
+__device__ inline void shard_emptied(const kma_dheap *H, uint32_t shard)
+{
+	uint64_t *mp = pmask(H, shard / shards_per_xcd(H));
+	uint64_t bit = shard_bit(H, shard);
+	f_and(mp, ~bit);
+	order_prior();
+	if ((uint32_t)ld(pool(H, shard)) != 0)
+		f_or(mp, bit);
+}
 
 /* Push never gives up: a failed CAS means another push or pop succeeded,
  * so the pool as a whole progresses; back-off spreads the retries. */
 __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb)
 {
 	uint64_t *hp = pool(H, shard);
-	/* the XCD count is raised before the superblock becomes poppable */
-	fadd(pcount(H, shard / shards_per_xcd(H)), (uint64_t)1);
 	uint64_t h = ld(hp);
 	uint32_t bo = 1;
 	for (;;) {
@@ -323,6 +344,8 @@ __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb
 			__builtin_amdgcn_s_sleep(1);
 		bo = bo < 16u ? 2u * bo : 16u;
 	}
+	if ((uint32_t)h == 0)
+		f_or(pmask(H, shard / shards_per_xcd(H)), shard_bit(H, shard));
 	fadd(hp + 1, (uint64_t)1);
 }
 
@@ -338,14 +361,17 @@ __device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32
 			return false;
 		}
 		uint32_t top = (uint32_t)h;
-		if (top == 0)
+		if (top == 0) {
+			shard_emptied(H, shard);
 			return false;
+		}
 		uint32_t nx = ld(pnext(H, top - 1u));
 		uint64_t nh = (((h >> 32) + 1ull) << 32) | (uint64_t)nx;
 		uint64_t seen = cas(hp, h, nh);
 		if (seen == h) {
 			fsub(hp + 1, (uint64_t)1);
-			fsub(pcount(H, shard / shards_per_xcd(H)), (uint64_t)1);
+			if (nx == 0)
+				shard_emptied(H, shard);
 			*out = top - 1u;
 			return true;
 		}
@@ -356,44 +382,44 @@ __device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32
 	}
 }
 
-/* Pop from shards first, first + ((start + i) % n) for i < n: their heads
- * are read eight at a time (one round trip per eight shards) and only the
- * non-empty ones are popped. */
-__device__ inline bool pool_pop_range(const kma_dheap *H, uint32_t first, uint32_t n, uint32_t start, uint32_t *out)
+/* Pop from the shards of XCD y whose hint bits are in `bits`, starting at
+ * the bit after position `start`. */
+__device__ inline bool pool_pop_bits(const kma_dheap *H, uint32_t y, uint64_t bits, uint32_t start, uint32_t *out)
 {
-	for (uint32_t i0 = 0; i0 < n; i0 += 8) {
-		uint64_t hv[8];
-#pragma unroll
-		for (uint32_t k = 0; k < 8; k++)
-			hv[k] = i0 + k < n ? ld(pool(H, first + (start + i0 + k) % n)) : 0ull;
-#pragma unroll
-		for (uint32_t k = 0; k < 8; k++)
-			if ((uint32_t)hv[k] != 0 && pool_pop_shard(H, first + (start + i0 + k) % n, out))
+	uint32_t S = shards_per_xcd(H);
+	uint32_t r = start % S;
+	uint64_t hi = bits & (~0ull << r), lo = bits & ~(~0ull << r);
+	for (int pass = 0; pass < 2; pass++) {
+		uint64_t m = pass ? lo : hi;
+		while (m) {
+			uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+			m &= m - 1ull;
+			if (pool_pop_shard(H, y * S + k, out))
 				return true;
+		}
 	}
 	return false;
 }
 
 /* Pop for XCD x: its own clean shards, then its own dirty shards, then the
- * other XCDs' clean shards.  XCDs whose count reads zero are skipped (the
- * eight counts are read in one round trip), so an exhausted pool fails
- * fast. */
+ * other XCDs' clean shards.  The eight shard masks are read in one round
+ * trip, so an exhausted pool fails fast. */
 __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t *out)
 {
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
-	uint64_t cnt[KMA_NXCD];
+	uint64_t all = S == 64 ? ~0ull : ((1ull << S) - 1ull);
+	uint64_t dirty = (1ull << D) - 1ull, clean = all & ~dirty;
+	uint64_t m[KMA_NXCD];
 #pragma unroll
 	for (uint32_t y = 0; y < KMA_NXCD; y++)
-		cnt[y] = ld(pcount(H, y));
-	if (cnt[x] != 0) {
-		if (pool_pop_range(H, x * S + D, S - D, ctx, out))
-			return true;
-		if (pool_pop_range(H, x * S, D, ctx, out))
-			return true;
-	}
+		m[y] = ld(pmask(H, y));
+	if (pool_pop_bits(H, x, m[x] & clean, D + ctx % (S - D), out))
+		return true;
+	if (pool_pop_bits(H, x, m[x] & dirty, ctx % D, out))
+		return true;
 	for (uint32_t j = 1; j < KMA_NXCD; j++) {
 		uint32_t y = (x + j) & (KMA_NXCD - 1u);
-		if (cnt[y] != 0 && pool_pop_range(H, y * S + D, S - D, ctx, out))
+		if (pool_pop_bits(H, y, m[y] & clean, D + ctx % (S - D), out))
 			return true;
 	}
 	return false;
@@ -710,7 +736,7 @@ __device__ inline void page_free(const kma_dheap *H, char *p)
  *   - superblocks start in the clean shards: clean shard j (of the
  *     C = 8 * (S - D) clean shards, S - D per XCD) holds superblocks j,
  *     j + C, j + 2C, ... (lowest on top); dirty shards start empty;
- *   - each XCD's pooled count is the sum over its clean shards;
+ *   - each XCD's shard mask has the bits of its non-empty clean shards;
  *   - a page-window summary bit starts set only for a word that does not
  *     exist or cannot hold a window of that level (a partial last word). */
 __device__ __forceinline__ uint64_t init_shard_len(uint64_t j, uint64_t nsb, uint64_t C)
@@ -732,10 +758,11 @@ __device__ inline void init_item(const kma_dheap *H, uint64_t i)
 		hp[1] = init_shard_len(i, nsb, C);
 	}
 	if (i < KMA_NXCD) {
-		uint64_t n = 0;
+		uint64_t m = 0;
 		for (uint64_t j = i * K; j < (i + 1) * K; j++)
-			n += init_shard_len(j, nsb, C);
-		*pcount(H, (uint32_t)i) = n;
+			if (init_shard_len(j, nsb, C))
+				m |= (uint64_t)1 << (D + (j - i * K));
+		*pmask(H, (uint32_t)i) = m;
 	}
 	uint64_t ns = psum_words(H), nwords = (H->npages + 63) >> 6;
 	if (H->npages && i < KMA_PAGE_LEVELS * ns) {

@@ -66,8 +66,8 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 	}
 	if (cfg->npool == 0)
 		cfg->npool = cfg->nctx * 2 < 256 ? cfg->nctx * 2 : 256;
-	if (!is_pow2(cfg->npool) || cfg->npool < 16 || cfg->npool > 65536) {
-		snprintf(err, errlen, "native layout: npool %u must be a power of two in 16..65536", cfg->npool);
+	if (!is_pow2(cfg->npool) || cfg->npool < 16 || cfg->npool > 512) {
+		snprintf(err, errlen, "native layout: npool %u must be a power of two in 16..512", cfg->npool);
 		return -1;
 	}
 	if (cfg->page_log2 == 0)

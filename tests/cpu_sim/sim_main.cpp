@@ -88,18 +88,13 @@ static int canonical_problems(kma_dheap *H)
 	}
 	if (n != H->nsb)
 		probs++;
-	/* each XCD's pooled count equals what its shards hold */
+	/* every non-empty shard has its hint bit set */
 	uint32_t S = H->npool / KMA_NXCD;
-	for (uint32_t x = 0; x < KMA_NXCD; x++) {
-		uint64_t held = 0;
-		for (uint32_t s = x * S; s < (x + 1) * S; s++)
-			held += pool(H, s)[1];
-		if (*pcount(H, x) != held) {
-			fprintf(stderr, "xcd %u count %llu holds %llu\n", x, (unsigned long long)*pcount(H, x),
-				(unsigned long long)held);
+	for (uint32_t s = 0; s < H->npool; s++)
+		if ((uint32_t)pool(H, s)[0] != 0 && !(*pmask(H, s / S) & ((uint64_t)1 << (s % S)))) {
+			fprintf(stderr, "shard %u non-empty but its hint bit is clear\n", s);
 			probs++;
 		}
-	}
 	return probs;
 }
 

@@ -53,7 +53,7 @@ class NatHeap:
         ns = (nwords + 63) // 64
         return {
             "stats": arr(h["off_stats"], 8, "<u8"),
-            "pcount": arr(int(h["off_stats"]) + 256, 8 * 16, "<u8")[::16],
+            "pmask": arr(int(h["off_stats"]) + 256, 8 * 16, "<u8")[::16],
             "psum": arr(h["off_psum"], PAGE_LEVELS * ns, "<u8").reshape(PAGE_LEVELS, ns),
             "state": arr(h["off_state"], self.nsb, "<u8"),
             "next": arr(h["off_next"], self.nsb, "<u4"),
@@ -102,10 +102,9 @@ class NatHeap:
             probs.append("pool holds %d superblocks (%d distinct), expected %d"
                          % (len(seen), len(set(seen)), self.nsb))
         per = self.npool // 8
-        for x in range(8):
-            held = int(m["pool"][x * per:(x + 1) * per, 1].sum())
-            if int(m["pcount"][x]) != held:
-                probs.append("xcd %d pooled count %d, shards hold %d" % (x, int(m["pcount"][x]), held))
+        for sh in range(self.npool):
+            if int(m["pool"][sh, 0]) & 0xFFFFFFFF and not (int(m["pmask"][sh // per]) >> (sh % per)) & 1:
+                probs.append("shard %d non-empty but its hint bit is clear" % sh)
         if self.npages and not np.array_equal(m["psum"], self.psum_initial()):
             probs.append("page-window summaries differ from the empty-heap image")
         return probs
