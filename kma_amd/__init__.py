@@ -31,6 +31,10 @@ EXPORTS = [
     "kma_stream_create", "kma_stream_destroy", "kma_stream_get_info",
     "kma_stream_trace", "kma_stream_results", "kma_stream_times", "kma_stream_run",
     "kma_stream_heap_config", "kma_stream_autotune",
+    "kma_pma_create", "kma_pma_destroy", "kma_pma_device", "kma_pma_reset", "kma_pma_head",
+    "kma_pma_run", "kma_queue_create", "kma_queue_destroy", "kma_queue_device", "kma_queue_reset",
+    "kma_queue_items", "kma_queue_run", "kma_list_create", "kma_list_destroy", "kma_list_device",
+    "kma_list_run_grow", "kma_list_run_get", "kma_list_clear", "kma_list_pages", "kma_tree_run",
 ]
 TABLE_POW2, TABLE_HALF = 0, 1
 TUNE_MAX = 64
@@ -205,6 +209,39 @@ def lib():
         L.kma_stream_times.argtypes = [vp, vp]
         L.kma_stream_run.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, P(StreamStats)]
         L.kma_stream_heap_config.argtypes = [vp, u64, u32, u32, u32, P(Config)]
+        i64 = ctypes.c_int64
+        L.kma_pma_create.argtypes = [u64, u32]
+        L.kma_pma_create.restype = vp
+        L.kma_pma_destroy.argtypes = [vp]
+        L.kma_pma_destroy.restype = None
+        L.kma_pma_device.argtypes = [vp]
+        L.kma_pma_device.restype = vp
+        L.kma_pma_reset.argtypes = [vp]
+        L.kma_pma_head.argtypes = [vp]
+        L.kma_pma_head.restype = u64
+        L.kma_pma_run.argtypes = [vp, vp, u64, ctypes.c_int, vp]
+        L.kma_queue_create.argtypes = [u64]
+        L.kma_queue_create.restype = vp
+        L.kma_queue_destroy.argtypes = [vp]
+        L.kma_queue_destroy.restype = None
+        L.kma_queue_device.argtypes = [vp]
+        L.kma_queue_device.restype = vp
+        L.kma_queue_reset.argtypes = [vp]
+        L.kma_queue_items.argtypes = [vp, vp, u64]
+        L.kma_queue_items.restype = i64
+        L.kma_queue_run.argtypes = [vp, u32, u64, u32, P(u64), vp]
+        L.kma_list_create.argtypes = [vp, u32, u64]
+        L.kma_list_create.restype = vp
+        L.kma_list_destroy.argtypes = [vp]
+        L.kma_list_destroy.restype = None
+        L.kma_list_device.argtypes = [vp]
+        L.kma_list_device.restype = vp
+        L.kma_list_run_grow.argtypes = [vp, vp, u64, u32, ctypes.c_int, vp]
+        L.kma_list_run_get.argtypes = [vp, vp, u64, vp]
+        L.kma_list_clear.argtypes = [vp, u64]
+        L.kma_list_pages.argtypes = [vp, vp, u64]
+        L.kma_list_pages.restype = i64
+        L.kma_tree_run.argtypes = [vp, vp, u64, vp, vp, u64, vp, vp, P(u64), P(u64)]
         L.kma_stream_autotune.argtypes = [vp, u64, u32, u64, u32, P(TuneResult)]
         _lib = L
     return _lib
@@ -413,3 +450,125 @@ class Stream:
         trials = [{k: getattr(t, k) for k, _ in TuneTrial._fields_} for t in r.trials[:r.ntrials]]
         return {"cfg": r.cfg, "config": config_dict(r.cfg), "nwg": r.nwg, "per_thread": r.per_thread,
                 "table": r.table, "ops_per_s": r.ops_per_s, "nulls": r.nulls, "trials": trials}
+
+
+class _Handle:
+    _destroy = None
+
+    def close(self):
+        if getattr(self, "_p", None):
+            getattr(lib(), self._destroy)(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Pma(_Handle):
+    """Bump allocator (kma_pma_create; the reference's pma_create)."""
+    _destroy = "kma_pma_destroy"
+
+    def __init__(self, nbytes, align=1):
+        self._p = lib().kma_pma_create(nbytes, align)
+        if not self._p:
+            raise KmaError("kma_pma_create: " + last_error())
+
+    def reset(self):
+        _check(lib().kma_pma_reset(self._p), "kma_pma_reset")
+
+    @property
+    def head(self):
+        return lib().kma_pma_head(self._p)
+
+    def run(self, sizes, serial=False):
+        """Data offsets of the blocks (UINT64_MAX for NULL)."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        out = np.zeros(len(sizes), dtype=np.uint64)
+        _check(lib().kma_pma_run(self._p, sizes.ctypes.data, len(sizes), 1 if serial else 0, out.ctypes.data),
+               "kma_pma_run")
+        return out
+
+
+class Queue(_Handle):
+    """MPMC pointer queue (kma_queue_create; the reference's clQueue_create)."""
+    _destroy = "kma_queue_destroy"
+
+    def __init__(self, capacity):
+        self._p = lib().kma_queue_create(capacity)
+        if not self._p:
+            raise KmaError("kma_queue_create: " + last_error())
+
+    def reset(self):
+        _check(lib().kma_queue_reset(self._p), "kma_queue_reset")
+
+    def items(self, maxn=1 << 20):
+        buf = np.zeros(maxn, dtype=np.uint64)
+        n = lib().kma_queue_items(self._p, buf.ctypes.data, maxn)
+        if n < 0:
+            raise KmaError("kma_queue_items: " + last_error())
+        return buf[:min(n, maxn)]
+
+    def run(self, mode, nwi, rounds=0):
+        """(item base address, dequeue order for mode 2 or None)"""
+        base = ctypes.c_uint64()
+        order = np.zeros(max(nwi, 1), dtype=np.int64) if mode == 2 else None
+        _check(lib().kma_queue_run(self._p, mode, nwi, rounds, ctypes.byref(base),
+                                   order.ctypes.data if order is not None else None), "kma_queue_run")
+        return base.value, (order[:nwi] if order is not None else None)
+
+
+class ArrayList(_Handle):
+    """Growable array list over a native heap (kma_list_create; the
+    reference's clArrayList_create)."""
+    _destroy = "kma_list_destroy"
+
+    def __init__(self, heap, obj_size, max_pages=1 << 16):
+        self._p = lib().kma_list_create(heap._h, obj_size, max_pages)
+        if not self._p:
+            raise KmaError("kma_list_create: " + last_error())
+        self.obj_size = obj_size
+
+    def grow(self, objs, block=256, local=True):
+        objs = np.ascontiguousarray(objs, dtype=np.uint32)
+        out = np.zeros(len(objs), dtype=np.uint64)
+        _check(lib().kma_list_run_grow(self._p, objs.ctypes.data, len(objs), block, 1 if local else 0,
+                                       out.ctypes.data), "kma_list_run_grow")
+        return out
+
+    def get(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        out = np.zeros(len(idx), dtype=np.uint64)
+        _check(lib().kma_list_run_get(self._p, idx.ctypes.data, len(idx), out.ctypes.data), "kma_list_run_get")
+        return out
+
+    def clear(self, nwi=1):
+        _check(lib().kma_list_clear(self._p, nwi), "kma_list_clear")
+
+    def pages(self, maxn=1 << 16):
+        buf = np.zeros(maxn, dtype=np.uint64)
+        n = lib().kma_list_pages(self._p, buf.ctypes.data, maxn)
+        if n < 0:
+            raise KmaError("kma_list_pages: " + last_error())
+        return buf[:min(n, maxn)]
+
+
+def tree_run(heap, keys, query):
+    """Concurrent kma_tree_add of keys, kma_tree_get of query, in-order walk;
+    returns (added, found, inorder keys, violations)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    query = np.ascontiguousarray(query, dtype=np.uint32)
+    added = np.zeros(len(keys), dtype=np.uint32)
+    found = np.zeros(len(query), dtype=np.uint32)
+    inorder = np.zeros(len(keys) + 1, dtype=np.uint32)
+    n = ctypes.c_uint64()
+    bad = ctypes.c_uint64()
+    _check(lib().kma_tree_run(heap._h, keys.ctypes.data, len(keys), added.ctypes.data, query.ctypes.data,
+                              len(query), found.ctypes.data, inorder.ctypes.data, ctypes.byref(n),
+                              ctypes.byref(bad)), "kma_tree_run")
+    return added, found, inorder[:min(n.value, len(keys) + 1)], bad.value

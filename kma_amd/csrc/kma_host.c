@@ -202,11 +202,6 @@ int kma_snapshot(const kma_heap *h, uint64_t offset, void *dst, uint64_t bytes)
 	return 0;
 }
 
-int kma_native_header_size(void)
-{
-	return (int)sizeof(nat_hdr);
-}
-
 /* ---------------- workloads ---------------- */
 int kma_trace_serial(kma_heap *h, const int64_t *ops, uint64_t n, uint64_t *out, const uint32_t *pat)
 {

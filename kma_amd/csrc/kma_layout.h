@@ -53,14 +53,15 @@ typedef struct nat_hdr {
 /* Validate `in`, fill defaults into *out and the header (metadata offsets;
  * off_data holds the metadata size until kma_native_place runs).
  * Returns 0, or -1 with a message in err. */
-int kma_native_layout(const kma_config *in, kma_config *out, nat_hdr *nh, char *err, size_t errlen);
+__attribute__((visibility("hidden"))) int kma_native_layout(const kma_config *in, kma_config *out, nat_hdr *nh,
+						  char *err, size_t errlen);
 
 /* Bytes to allocate for the heap (metadata + alignment slack + data). */
-uint64_t kma_native_alloc_bytes(const nat_hdr *nh);
+__attribute__((visibility("hidden"))) uint64_t kma_native_alloc_bytes(const nat_hdr *nh);
 
 /* Place superblock data and pages for a buffer at device address `base`,
  * so both are aligned absolutely to their sizes. */
-void kma_native_place(nat_hdr *nh, uint64_t base, uint64_t total);
+__attribute__((visibility("hidden"))) void kma_native_place(nat_hdr *nh, uint64_t base, uint64_t total);
 
 #ifdef __cplusplus
 }
