@@ -219,6 +219,25 @@ typedef struct kma_tune_result {
 int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, uint64_t null_limit,
 			uint32_t reps, kma_tune_result *out);
 
+/* ------------------------------------------------------------------ */
+/* Deterministic batch mode (reference layout)                          */
+/* ------------------------------------------------------------------ */
+/* Serve n mallocs as ONE parallel batch whose result is identical, address
+ * for address and byte for byte of the heap image, to malloc(sizes[0]), ...,
+ * malloc(sizes[n-1]) run one at a time in index order (kma.cl:279-303).  The
+ * heap must hold no live block and no attached class superblock (as after
+ * kma_create / kma_reset or kma_batch_free_det of the previous batch), and
+ * no request may fall in a 1-slot class (serialized, the reference blocks
+ * on the second such request).  out[i] = heap offset of request i's block,
+ * 0 for NULL.  *ms (optional) = device time of the batch, inputs already
+ * resident.  Returns the number of NULLs, or -1. */
+int64_t kma_batch_malloc_det(kma_heap *h, const uint32_t *sizes, uint64_t n, uint64_t *out, double *ms);
+
+/* Free the blocks of the preceding kma_batch_malloc_det batch (offs = its
+ * out array, n its length) with the result of free(offs[0]), ...,
+ * free(offs[n-1]) run one at a time in index order (kma.cl:305-373). */
+int kma_batch_free_det(kma_heap *h, const uint64_t *offs, uint64_t n, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ EXPORTS = [
     "kma_pma_run", "kma_queue_create", "kma_queue_destroy", "kma_queue_device", "kma_queue_reset",
     "kma_queue_items", "kma_queue_run", "kma_list_create", "kma_list_destroy", "kma_list_device",
     "kma_list_run_grow", "kma_list_run_get", "kma_list_clear", "kma_list_pages", "kma_tree_run",
+    "kma_batch_malloc_det", "kma_batch_free_det",
 ]
 TABLE_POW2, TABLE_HALF = 0, 1
 TUNE_MAX = 64
@@ -242,6 +243,9 @@ def lib():
         L.kma_list_pages.argtypes = [vp, vp, u64]
         L.kma_list_pages.restype = i64
         L.kma_tree_run.argtypes = [vp, vp, u64, vp, vp, u64, vp, vp, P(u64), P(u64)]
+        L.kma_batch_malloc_det.argtypes = [vp, vp, u64, vp, P(ctypes.c_double)]
+        L.kma_batch_malloc_det.restype = i64
+        L.kma_batch_free_det.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         L.kma_stream_autotune.argtypes = [vp, u64, u32, u64, u32, P(TuneResult)]
         _lib = L
     return _lib
@@ -349,6 +353,26 @@ class Heap:
                "kma_trace_serial")
         return out
 
+    def batch_malloc_det(self, sizes):
+        """Deterministic batch malloc (reference layout): (offsets, NULL count,
+        device ms)."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        out = np.zeros(len(sizes), dtype=np.uint64)
+        ms = ctypes.c_double()
+        nulls = lib().kma_batch_malloc_det(self._h, sizes.ctypes.data, len(sizes), out.ctypes.data,
+                                           ctypes.byref(ms))
+        if nulls < 0:
+            raise KmaError("kma_batch_malloc_det: " + last_error())
+        return out, nulls, ms.value
+
+    def batch_free_det(self, offs):
+        """Free the preceding deterministic batch in index order; device ms."""
+        offs = np.ascontiguousarray(offs, dtype=np.uint64)
+        ms = ctypes.c_double()
+        _check(lib().kma_batch_free_det(self._h, offs.ctypes.data, len(offs), ctypes.byref(ms)),
+               "kma_batch_free_det")
+        return ms.value
+
     def run(self, mode, nwi, rounds, form=FORM_PLAIN, block=64, fixed=64, seed=1,
             round0=0, warmup=0, variant=0):
         w = Workload(mode, form, nwi, block, rounds, round0, fixed, seed, warmup, 0, variant)
@@ -450,6 +474,27 @@ class Stream:
         trials = [{k: getattr(t, k) for k, _ in TuneTrial._fields_} for t in r.trials[:r.ntrials]]
         return {"cfg": r.cfg, "config": config_dict(r.cfg), "nwg": r.nwg, "per_thread": r.per_thread,
                 "table": r.table, "ops_per_s": r.ops_per_s, "nulls": r.nulls, "trials": trials}
+
+
+def splitmix(seed, gid, k):
+    """splitmix64 finaliser of seed*K1 + gid*K2 + k*K3 (vectorised over gid)."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.asarray(gid, dtype=np.uint64)
+             * np.uint64(0xBF58476D1CE4E5B9) + np.uint64(k) * np.uint64(0x94D049BB133111EB)) & M
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def loguniform_sizes(n, lo_log2, hi_log2, seed):
+    """n sizes log-uniform over [2^lo_log2, 2^hi_log2): exponent e uniform,
+    then uniform within the octave, integer only."""
+    gid = np.arange(n, dtype=np.uint64)
+    e = np.uint64(lo_log2) + splitmix(seed, gid, 0) % np.uint64(hi_log2 - lo_log2)
+    m = (splitmix(seed, gid, 1) >> np.uint64(16)) % (np.uint64(1) << e)
+    return ((np.uint64(1) << e) + m).astype(np.uint32)
 
 
 class _Handle:

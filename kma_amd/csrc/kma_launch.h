@@ -69,6 +69,29 @@ int kmad_tree_run(void *heap, void *tree, const uint32_t *d_keys, uint64_t n, ui
 		  const uint32_t *d_query, uint64_t nq, uint32_t *d_found, void *d_stack,
 		  uint32_t *d_inorder, uint64_t *d_res);
 
+/* Deterministic batch mode, reference layout (kma_detbatch.hip). */
+int kmad_det_class(void *heap, const uint32_t *d_sizes, uint64_t n, int8_t *d_cls, uint32_t *d_cnt,
+		   uint32_t *d_err);
+int kmad_det_scan(uint32_t *d_tab, uint64_t nrows, uint32_t ncol, uint32_t *d_tot);
+int kmad_det_rank(void *heap, const int8_t *d_cls, uint64_t n, const uint32_t *d_off, uint32_t *d_rank,
+		  uint32_t *d_evcnt);
+int kmad_det_coff(void *heap, const uint32_t *d_tot, uint32_t *d_coff);
+int kmad_det_events(void *heap, const int8_t *d_cls, const uint32_t *d_rank, uint64_t n,
+		    const uint32_t *d_evoff, const uint32_t *d_coff, uint32_t *d_table, uint32_t *d_evc,
+		    uint32_t *d_evq);
+int kmad_det_pops(void *heap, const uint32_t *d_nev, uint32_t *d_popped, uint32_t *d_npop);
+int kmad_det_refill(void *heap, uint64_t maxev, const uint32_t *d_npop, const uint32_t *d_popped,
+		    const uint32_t *d_evc, const uint32_t *d_evq, const uint32_t *d_tot);
+int kmad_det_claim(void *heap, const int8_t *d_cls, const uint32_t *d_rank, uint64_t n,
+		   const uint32_t *d_coff, const uint32_t *d_table, const uint32_t *d_npop,
+		   const uint32_t *d_popped, uint64_t *d_out);
+int kmad_detf_min(void *heap, const uint64_t *d_offs, uint64_t n, uint32_t *d_minblk);
+int kmad_detf_words(void *heap, const uint64_t *d_offs, uint64_t n, const uint32_t *d_minblk,
+		    uint32_t *d_evcnt, uint32_t *d_last_sb);
+int kmad_detf_order(void *heap, const uint64_t *d_offs, uint64_t n, const uint32_t *d_minblk,
+		    const uint32_t *d_evoff, const uint32_t *d_last_sb, uint32_t *d_order);
+int kmad_detf_enqueue(void *heap, const uint32_t *d_order, const uint32_t *d_ne);
+
 #ifdef __cplusplus
 }
 #endif
