@@ -786,6 +786,20 @@ __host__ __device__ inline uint64_t init_items(uint64_t nsb, uint32_t npool, uin
 
 } /* namespace kma_nat */
 
+/* Superblocks currently in the pool (free for any class): the sum of the
+ * shard counts, read by a full wavefront (all 64 lanes call it; every lane
+ * gets the sum).  A snapshot: pushes and pops in flight may be missed. */
+__device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
+{
+	using namespace kma_nat;
+	uint64_t s = 0;
+	for (uint32_t k = lane(); k < H->npool; k += 64)
+		s += ld(pool(H, k) + 1);
+	for (int o = 32; o > 0; o >>= 1)
+		s += __shfl_xor(s, o);
+	return s;
+}
+
 /* Allocate `size` bytes; NULL for size 0 or when the heap cannot serve it. */
 __device__ inline void *kma_malloc(kma_dheap *H, size_t size)
 {

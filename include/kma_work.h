@@ -220,6 +220,63 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 			uint32_t reps, kma_tune_result *out);
 
 /* ------------------------------------------------------------------ */
+/* Serving loop                                                         */
+/* ------------------------------------------------------------------ */
+/* Jobs arrive as a Poisson process and run for a log-normal number of
+ * steps; each step appends one block of block_bytes to every live job, and
+ * a job's blocks are all freed when it completes.  One device-resident
+ * persistent kernel admits arrived jobs (oldest first) while the heap's use
+ * is below watermark_pct and a job slot is free, grows every live job by one
+ * block per step with one work-group collective malloc per 256 slots, and
+ * frees completed jobs' blocks with the whole grid.  A job whose malloc
+ * returns NULL waits a step.  Job j's result is the sum of the 64-bit
+ * patterns its blocks carried, which depends on (j, its length) only. */
+typedef struct kma_serve_spec {
+	double rate_per_ms;     /* mean arrivals per millisecond */
+	double len_mu;          /* log-normal length: exp(mu + sigma z) steps */
+	double len_sigma;
+	double duration_ms;     /* wall-clock length of the run */
+	uint32_t max_len;       /* lengths are capped here (block-table width) */
+	uint32_t block_bytes;   /* appended per live job per step */
+	uint32_t watermark_pct; /* admit while heap use is below this percentage */
+	uint32_t max_live;      /* job slots (multiple of 256) */
+	uint32_t max_admit;     /* admissions per step (0: 1024) */
+	uint32_t nwg;           /* work-groups (0: largest co-resident grid) */
+	uint64_t seed;
+} kma_serve_spec;
+
+typedef struct kma_serve_stats {
+	double run_ms;          /* device wall clock from start to the stop decision */
+	double kernel_ms;       /* HIP-event time of the whole launch (drain included) */
+	uint64_t jobs;          /* jobs in the trace */
+	uint64_t truncated;     /* lengths capped at max_len */
+	uint64_t steps, admitted, done, drained; /* drained: still live at the stop */
+	uint64_t mallocs, frees, stalls, drain_frees, bad;
+	double ops_per_s;       /* (mallocs + frees) / run_ms */
+	double lat_p50_ns, lat_p99_ns, lat_max_ns; /* malloc latency (collective call) */
+	double wait_p50_us, wait_p99_us;           /* arrival -> admission, admitted jobs */
+	double peak_used_frac;  /* most superblocks out of the pool at a step start */
+	uint32_t nwg;
+	uint32_t aborted;       /* a barrier watchdog fired */
+} kma_serve_stats;
+
+/* Run the serving loop on a native heap.  jobs == NULL: the whole trace;
+ * otherwise only the listed job ids, all arriving at the start (with the
+ * lengths they have in the whole trace).  results (optional, nresults
+ * entries) gets the result of each job of the run (0 if it did not
+ * complete): for jobs == NULL indexed by job id, else by position in jobs. */
+int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs, uint64_t njobs,
+		  uint64_t *results, uint64_t nresults, kma_serve_stats *out);
+
+/* The serving trace on the host (no device needed): arrival ticks of the
+ * device wall clock and lengths of up to `max` jobs; returns the number of
+ * jobs in the trace. */
+int64_t kma_serve_trace(const kma_serve_spec *spec, uint64_t *arrival, uint32_t *lengths, uint64_t max);
+
+/* Lengths of the listed jobs (host only). */
+int kma_serve_lengths(const kma_serve_spec *spec, const uint64_t *jobs, uint64_t n, uint32_t *lengths);
+
+/* ------------------------------------------------------------------ */
 /* Deterministic batch mode (reference layout)                          */
 /* ------------------------------------------------------------------ */
 /* Serve n mallocs as ONE parallel batch whose result is identical, address

@@ -35,7 +35,7 @@ EXPORTS = [
     "kma_pma_run", "kma_queue_create", "kma_queue_destroy", "kma_queue_device", "kma_queue_reset",
     "kma_queue_items", "kma_queue_run", "kma_list_create", "kma_list_destroy", "kma_list_device",
     "kma_list_run_grow", "kma_list_run_get", "kma_list_clear", "kma_list_pages", "kma_tree_run",
-    "kma_batch_malloc_det", "kma_batch_free_det",
+    "kma_batch_malloc_det", "kma_batch_free_det", "kma_serve_run", "kma_serve_trace", "kma_serve_lengths",
 ]
 TABLE_POW2, TABLE_HALF = 0, 1
 TUNE_MAX = 64
@@ -163,6 +163,33 @@ def config_dict(c):
             "npages": c.npages}
 
 
+class ServeSpec(ctypes.Structure):
+    _fields_ = [
+        ("rate_per_ms", ctypes.c_double), ("len_mu", ctypes.c_double), ("len_sigma", ctypes.c_double),
+        ("duration_ms", ctypes.c_double), ("max_len", ctypes.c_uint32), ("block_bytes", ctypes.c_uint32),
+        ("watermark_pct", ctypes.c_uint32), ("max_live", ctypes.c_uint32), ("max_admit", ctypes.c_uint32),
+        ("nwg", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("run_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double)] + [
+        (k, ctypes.c_uint64) for k in ("jobs", "truncated", "steps", "admitted", "done", "drained", "mallocs",
+                                       "frees", "stalls", "drain_frees", "bad")] + [
+        (k, ctypes.c_double) for k in ("ops_per_s", "lat_p50_ns", "lat_p99_ns", "lat_max_ns", "wait_p50_us",
+                                       "wait_p99_us", "peak_used_frac")] + [
+        ("nwg", ctypes.c_uint32), ("aborted", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def serve_spec(rate_per_ms=2000.0, len_mu=6.0, len_sigma=1.0, duration_ms=10000.0, max_len=8192,
+               block_bytes=16384, watermark_pct=90, max_live=32768, max_admit=0, nwg=0, seed=4):
+    return ServeSpec(rate_per_ms, len_mu, len_sigma, duration_ms, max_len, block_bytes, watermark_pct, max_live,
+                     max_admit, nwg, seed)
+
+
 class KmaError(RuntimeError):
     pass
 
@@ -246,6 +273,10 @@ def lib():
         L.kma_batch_malloc_det.argtypes = [vp, vp, u64, vp, P(ctypes.c_double)]
         L.kma_batch_malloc_det.restype = i64
         L.kma_batch_free_det.argtypes = [vp, vp, u64, P(ctypes.c_double)]
+        L.kma_serve_run.argtypes = [vp, P(ServeSpec), vp, u64, vp, u64, P(ServeStats)]
+        L.kma_serve_trace.argtypes = [P(ServeSpec), vp, vp, u64]
+        L.kma_serve_trace.restype = i64
+        L.kma_serve_lengths.argtypes = [P(ServeSpec), vp, u64, vp]
         L.kma_stream_autotune.argtypes = [vp, u64, u32, u64, u32, P(TuneResult)]
         _lib = L
     return _lib
@@ -617,3 +648,49 @@ def tree_run(heap, keys, query):
                               len(query), found.ctypes.data, inorder.ctypes.data, ctypes.byref(n),
                               ctypes.byref(bad)), "kma_tree_run")
     return added, found, inorder[:min(n.value, len(keys) + 1)], bad.value
+
+
+def serve_run(heap, spec, jobs=None, nresults=0):
+    """Run the serving loop (kma_serve_run): (stats dict, results array)."""
+    st = ServeStats()
+    jp, nj = None, 0
+    if jobs is not None:
+        jobs = np.ascontiguousarray(jobs, dtype=np.uint64)
+        jp, nj = jobs.ctypes.data, len(jobs)
+        nresults = nj
+    res = np.zeros(max(nresults, 1), dtype=np.uint64)
+    _check(lib().kma_serve_run(heap._h, ctypes.byref(spec), jp, nj, res.ctypes.data if nresults else None,
+                               nresults, ctypes.byref(st)), "kma_serve_run")
+    return st.as_dict(), res[:nresults]
+
+
+def serve_trace(spec, maxn=1 << 26):
+    """(arrival ticks, lengths) of the serving trace (host only)."""
+    n = lib().kma_serve_trace(ctypes.byref(spec), None, None, 0)
+    if n < 0:
+        raise KmaError("kma_serve_trace: " + last_error())
+    n = min(n, maxn)
+    arr = np.zeros(max(n, 1), dtype=np.uint64)
+    ln = np.zeros(max(n, 1), dtype=np.uint32)
+    lib().kma_serve_trace(ctypes.byref(spec), arr.ctypes.data, ln.ctypes.data, n)
+    return arr[:n], ln[:n]
+
+
+def serve_lengths(spec, jobs):
+    jobs = np.ascontiguousarray(jobs, dtype=np.uint64)
+    out = np.zeros(max(len(jobs), 1), dtype=np.uint32)
+    _check(lib().kma_serve_lengths(ctypes.byref(spec), jobs.ctypes.data, len(jobs), out.ctypes.data),
+           "kma_serve_lengths")
+    return out[:len(jobs)]
+
+
+def serve_pattern(job, k):
+    """The 64-bit pattern the serving loop stamps into block k of job `job`
+    (vectorised over k)."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(job) * np.uint64(0x9E3779B97F4A7C15) + np.asarray(k, dtype=np.uint64)
+             * np.uint64(0xBF58476D1CE4E5B9) + np.uint64(0x94D049BB133111EB)) & M
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))

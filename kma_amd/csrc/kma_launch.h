@@ -69,6 +69,39 @@ int kmad_tree_run(void *heap, void *tree, const uint32_t *d_keys, uint64_t n, ui
 		  const uint32_t *d_query, uint64_t nq, uint32_t *d_found, void *d_stack,
 		  uint32_t *d_inorder, uint64_t *d_res);
 
+/* Serving loop (kma_serve.hip).  ctl: KMA_SV_CTL_WORDS u64. */
+#define KMA_SV_CTL_WORDS 256
+#define KMA_SV_STATS 160 /* ctl index of the counters */
+#define KMA_SV_NHIST 65536
+typedef struct kma_serve_args {
+	void *heap;               /* kma_dheap */
+	const uint64_t *arrival;  /* per job: arrival tick after the loop's start */
+	const uint32_t *length;   /* per job: steps */
+	const uint64_t *job_id;   /* per job: the id its patterns use (NULL: its index) */
+	uint64_t njobs;
+	uint32_t max_live;        /* job slots, multiple of 256 */
+	uint32_t max_len;         /* block-table width */
+	uint32_t block_bytes;
+	uint32_t watermark_pct;
+	uint32_t max_admit;       /* admissions per step */
+	uint32_t pad;
+	uint64_t duration_ticks;
+	int64_t *slot_job;        /* -1: free */
+	uint32_t *slot_steps;
+	uint64_t *table;          /* [max_live][max_len] block addresses */
+	uint32_t *free_slots;     /* stack, max_live entries */
+	uint64_t *ctl;
+	uint64_t *done;           /* [max_live][4]: slot, job, first, count */
+	uint64_t *result;         /* per job: sum of its blocks' patterns */
+	uint64_t *job_admit;      /* per job: admission tick + 1 (0: never) */
+	uint64_t *job_done;       /* per job: completion tick + 1 (0: never) */
+	uint64_t *lat_hist;       /* KMA_SV_NHIST buckets of 1 wall-clock tick */
+} kma_serve_args;
+
+/* Largest co-resident grid for the serving kernel (256-thread groups). */
+int kmad_serve_max_groups(uint32_t *out);
+int kmad_serve_launch(const kma_serve_args *a, uint32_t nwg, void *stream);
+
 /* Deterministic batch mode, reference layout (kma_detbatch.hip). */
 int kmad_det_class(void *heap, const uint32_t *d_sizes, uint64_t n, int8_t *d_cls, uint32_t *d_cnt,
 		   uint32_t *d_err);

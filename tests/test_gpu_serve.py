@@ -1,0 +1,73 @@
+"""The device-resident serving loop (kma_serve_run) on the GPU: continuous
+admission by watermark, one block per live job per step, completed jobs'
+blocks freed.  Each job's result is the sum of the patterns its blocks
+carried, so it must equal the closed form sum_k pattern(j, k) and be the
+same when the job runs alone as in the mix (tolerance: exact)."""
+import numpy as np
+import pytest
+
+import kma_amd
+from tests.natimg import NatHeap
+
+pytestmark = pytest.mark.gpu
+
+
+def serve_heap(sblocks=256):
+    # 1 MiB superblocks of 64 x 16 KiB blocks
+    return kma_amd.Heap(layout=kma_amd.LAYOUT_NATIVE, sb_log2=20, sblocks=sblocks, classes=[16384])
+
+
+def expected(spec, jobs):
+    ln = kma_amd.serve_lengths(spec, jobs)
+    out = np.zeros(len(jobs), dtype=np.uint64)
+    for i, (j, L) in enumerate(zip(jobs, ln)):
+        with np.errstate(over="ignore"):
+            out[i] = kma_amd.serve_pattern(int(j), np.arange(int(L), dtype=np.uint64)).sum(dtype=np.uint64)
+    return out
+
+
+def check_stats(st):
+    assert not st["aborted"]
+    assert st["bad"] == 0
+    assert st["mallocs"] == st["frees"] + st["drain_frees"], st
+    assert st["admitted"] == st["done"] + st["drained"], st
+
+
+def test_serve_mix_and_alone():
+    print("serve mix", flush=True)
+    spec = kma_amd.serve_spec(rate_per_ms=40.0, len_mu=3.5, len_sigma=1.0, duration_ms=120.0, max_len=1024,
+                              max_live=2048, nwg=64, seed=4)
+    with serve_heap() as h:
+        st, res = kma_amd.serve_run(h, spec, nresults=1 << 20)
+        check_stats(st)
+        assert st["done"] > 100, st
+        done = np.nonzero(res)[0].astype(np.uint64)
+        assert len(done) == st["done"]
+        assert np.array_equal(res[done.astype(np.int64)], expected(spec, done))
+        h.trim()
+        assert NatHeap(h).canonical_problems() == []
+        # the same jobs alone
+        pick = done[[0, len(done) // 2, len(done) - 1]]
+        for j in pick:
+            st1, r1 = kma_amd.serve_run(h, spec, jobs=[int(j)])
+            check_stats(st1)
+            assert st1["done"] == 1 and r1[0] == res[int(j)], (j, r1, res[int(j)])
+        h.trim()
+        assert NatHeap(h).canonical_problems() == []
+
+
+def test_serve_memory_pressure():
+    """A heap far smaller than the demand: admission stops at the
+    watermark, jobs stall on NULL and resume; results stay exact."""
+    print("serve pressure", flush=True)
+    spec = kma_amd.serve_spec(rate_per_ms=200.0, len_mu=4.0, len_sigma=1.0, duration_ms=80.0, max_len=1024,
+                              max_live=4096, nwg=64, watermark_pct=80, seed=5)
+    with serve_heap(sblocks=16) as h:  # 16 MiB: 1,024 blocks
+        st, res = kma_amd.serve_run(h, spec, nresults=1 << 20)
+        check_stats(st)
+        assert st["admitted"] < st["jobs"], st  # the watermark held jobs back
+        done = np.nonzero(res)[0].astype(np.uint64)
+        assert len(done) > 0
+        assert np.array_equal(res[done.astype(np.int64)], expected(spec, done))
+        h.trim()
+        assert NatHeap(h).canonical_problems() == []
