@@ -13,7 +13,9 @@ device from the seed; nothing crosses PCIe in the timed region.
 Also measured (key "batched_stream"): the batched-stream workload, a
 host-generated trace of 4,096 uneven batches already resident in HBM, run
 as one launch per batch (plain) and as one graph-captured persistent launch
-with the autotuned heap.
+with the autotuned heap.  Key "serving": the device-resident serving loop
+(10 s of Poisson arrivals on a 64 GiB heap) with malloc latency
+percentiles.
 
 Prints ONE JSON line (rank 0, N = 1).
 """
@@ -125,6 +127,43 @@ def bench_stream(reps):
     }
 
 
+# Serving loop: Poisson arrivals (2,000 jobs/ms), log-normal lengths
+# (mu 6, sigma 1 steps, capped at 8,192), one 16 KiB block per live job per
+# step, whole chain freed at completion, 64 GiB heap (1 MiB superblocks of
+# 64 blocks), admission below 90 % use, 10 s, seed 4.
+SERVE = dict(rate_per_ms=2000.0, len_mu=6.0, len_sigma=1.0, duration_ms=10000.0, max_len=8192,
+             block_bytes=16384, watermark_pct=90, max_live=32768, seed=4)
+SERVE_HEAP_SB = 65536  # x 1 MiB = 64 GiB
+
+
+def bench_serve(duration_ms):
+    import kma_amd
+
+    spec = kma_amd.serve_spec(**dict(SERVE, duration_ms=duration_ms))
+    with kma_amd.Heap(layout=kma_amd.LAYOUT_NATIVE, sb_log2=20, sblocks=SERVE_HEAP_SB, classes=[16384]) as h:
+        st, _ = kma_amd.serve_run(h, spec)
+    log("bench: serving loop %s" % {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
+    return {
+        "workload": "Poisson arrivals 2,000 jobs/ms, log-normal lengths (mu 6, sigma 1) capped at 8,192 steps, "
+                    "+16 KiB per live job per step, chain freed at completion, 64 GiB heap, admission below "
+                    "90%% use, %g ms, seed 4" % duration_ms,
+        "ops_per_s": st["ops_per_s"],
+        "run_ms": st["run_ms"],
+        "steps": st["steps"],
+        "jobs_arrived": st["jobs"],
+        "jobs_admitted": st["admitted"],
+        "jobs_done": st["done"],
+        "mallocs": st["mallocs"],
+        "frees": st["frees"],
+        "stalls": st["stalls"],
+        "malloc_latency_ns": {"p50": st["lat_p50_ns"], "p99": st["lat_p99_ns"], "max": st["lat_max_ns"]},
+        "admission_wait_us": {"p50": st["wait_p50_us"], "p99": st["wait_p99_us"]},
+        "peak_used_frac": st["peak_used_frac"],
+        "violations": int(st["bad"] + st["aborted"]),
+        "work_groups": st["nwg"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +171,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-ms", type=float, default=10000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every heap is device-local and this build runs on "
@@ -226,6 +267,8 @@ def main():
     }
     if not args.no_stream:
         out["batched_stream"] = bench_stream(5)
+    if not args.no_serve:
+        out["serving"] = bench_serve(args.serve_ms)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

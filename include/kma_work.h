@@ -229,8 +229,11 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
  * is below watermark_pct and a job slot is free, grows every live job by one
  * block per step with one work-group collective malloc per 256 slots, and
  * frees completed jobs' blocks with the whole grid.  A job whose malloc
- * returns NULL waits a step.  Job j's result is the sum of the 64-bit
- * patterns its blocks carried, which depends on (j, its length) only. */
+ * returns NULL waits a step; after a step with such stalls the most recently
+ * admitted jobs are preempted (blocks freed, job requeued ahead of new
+ * arrivals, restarting from step 0) until their blocks cover the shortfall.
+ * Job j's result is the sum of the 64-bit patterns its blocks carried in the
+ * run that completed it, which depends on (j, its length) only. */
 typedef struct kma_serve_spec {
 	double rate_per_ms;     /* mean arrivals per millisecond */
 	double len_mu;          /* log-normal length: exp(mu + sigma z) steps */
@@ -241,7 +244,7 @@ typedef struct kma_serve_spec {
 	uint32_t watermark_pct; /* admit while heap use is below this percentage */
 	uint32_t max_live;      /* job slots (multiple of 256) */
 	uint32_t max_admit;     /* admissions per step (0: 1024) */
-	uint32_t nwg;           /* work-groups (0: largest co-resident grid) */
+	uint32_t nwg;           /* work-groups (0: max(64, max_live / 256), co-resident) */
 	uint64_t seed;
 } kma_serve_spec;
 
@@ -252,6 +255,7 @@ typedef struct kma_serve_stats {
 	uint64_t truncated;     /* lengths capped at max_len */
 	uint64_t steps, admitted, done, drained; /* drained: still live at the stop */
 	uint64_t mallocs, frees, stalls, drain_frees, bad;
+	uint64_t preempted;     /* admissions ended by preemption (the job is requeued) */
 	double ops_per_s;       /* (mallocs + frees) / run_ms */
 	double lat_p50_ns, lat_p99_ns, lat_max_ns; /* malloc latency (collective call) */
 	double wait_p50_us, wait_p99_us;           /* arrival -> admission, admitted jobs */

@@ -175,7 +175,7 @@ class ServeSpec(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("run_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double)] + [
         (k, ctypes.c_uint64) for k in ("jobs", "truncated", "steps", "admitted", "done", "drained", "mallocs",
-                                       "frees", "stalls", "drain_frees", "bad")] + [
+                                       "frees", "stalls", "drain_frees", "bad", "preempted")] + [
         (k, ctypes.c_double) for k in ("ops_per_s", "lat_p50_ns", "lat_p99_ns", "lat_max_ns", "wait_p50_us",
                                        "wait_p99_us", "peak_used_frac")] + [
         ("nwg", ctypes.c_uint32), ("aborted", ctypes.c_uint32)]

@@ -96,6 +96,11 @@ typedef struct kma_serve_args {
 	uint64_t *job_admit;      /* per job: admission tick + 1 (0: never) */
 	uint64_t *job_done;       /* per job: completion tick + 1 (0: never) */
 	uint64_t *lat_hist;       /* KMA_SV_NHIST buckets of 1 wall-clock tick */
+	uint32_t *slot_seq;       /* per slot: admission sequence number */
+	uint64_t *lstack;         /* admission-ordered (seq << 32 | slot) entries */
+	uint64_t lstack_cap;      /* >= 2 * max_live */
+	uint64_t *requeue;        /* ring of preempted job ids */
+	uint64_t rq_cap;
 } kma_serve_args;
 
 /* Largest co-resident grid for the serving kernel (256-thread groups). */

@@ -192,7 +192,10 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 		set_err("serving loop: occupancy query failed");
 		return -1;
 	}
-	uint32_t nwg = sp.nwg ? sp.nwg : maxg;
+	/* default: one work-group per 256 job slots (the barriers' cost grows
+	 * with the grid), at least 64, within the co-resident limit */
+	uint32_t want = sp.max_live / 256u < 64u ? 64u : sp.max_live / 256u;
+	uint32_t nwg = sp.nwg ? sp.nwg : (want < maxg ? want : maxg);
 	if (nwg > maxg) {
 		free(arr);
 		free(len);
@@ -210,17 +213,18 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	a.watermark_pct = sp.watermark_pct;
 	a.max_admit = sp.max_admit;
 	a.duration_ticks = (uint64_t)(sp.duration_ms * ticks_per_ms);
-	void *bufs[13] = {0};
-	uint64_t nn = n ? n : 1;
-	uint64_t sizes[13] = {nn * 8, nn * 4, (uint64_t)sp.max_live * 8, (uint64_t)sp.max_live * 4,
+	void *bufs[16] = {0};
+	uint64_t nn = n ? n : 1, rq = (uint64_t)sp.max_live * 4;
+	uint64_t sizes[16] = {nn * 8, nn * 4, (uint64_t)sp.max_live * 8, (uint64_t)sp.max_live * 4,
 			      (uint64_t)sp.max_live * sp.max_len * 8, (uint64_t)sp.max_live * 4, KMA_SV_CTL_WORDS * 8,
-			      (uint64_t)sp.max_live * 32, nn * 8, nn * 8, nn * 8, KMA_SV_NHIST * 8, nn * 8};
+			      (uint64_t)sp.max_live * 32, nn * 8, nn * 8, nn * 8, KMA_SV_NHIST * 8,
+			      (uint64_t)sp.max_live * 4, (uint64_t)sp.max_live * 16, rq * 8, nn * 8};
 	int rc = -1;
 	hipStream_t st = NULL;
 	hipEvent_t e0 = NULL, e1 = NULL;
 	uint64_t *hist = NULL, *adm = NULL, *waits = NULL;
 	uint32_t *fs = NULL;
-	for (int i = 0; i < (jobs ? 13 : 12); i++)
+	for (int i = 0; i < (jobs ? 16 : 15); i++)
 		if (hipMalloc(&bufs[i], sizes[i]) != hipSuccess) {
 			set_err("serving loop: device allocation of %llu bytes failed", (unsigned long long)sizes[i]);
 			goto done;
@@ -237,7 +241,12 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	a.job_admit = (uint64_t *)bufs[9];
 	a.job_done = (uint64_t *)bufs[10];
 	a.lat_hist = (uint64_t *)bufs[11];
-	a.job_id = jobs ? (const uint64_t *)bufs[12] : NULL;
+	a.slot_seq = (uint32_t *)bufs[12];
+	a.lstack = (uint64_t *)bufs[13];
+	a.lstack_cap = (uint64_t)sp.max_live * 2;
+	a.requeue = (uint64_t *)bufs[14];
+	a.rq_cap = rq;
+	a.job_id = jobs ? (const uint64_t *)bufs[15] : NULL;
 	fs = (uint32_t *)malloc((size_t)sp.max_live * 4);
 	if (!fs)
 		goto done;
@@ -246,7 +255,8 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	uint64_t ctl0[KMA_SV_CTL_WORDS];
 	memset(ctl0, 0, sizeof(ctl0));
 	ctl0[SV_C_NFREE] = sp.max_live;
-	if ((jobs && hipMemcpy(bufs[12], jobs, n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
+	if ((jobs && hipMemcpy(bufs[15], jobs, n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
+	    hipMemset(bufs[12], 0, sizes[12]) != hipSuccess ||
 	    (n && hipMemcpy(bufs[0], arr, n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
 	    (n && hipMemcpy(bufs[1], len, n * 4, hipMemcpyHostToDevice) != hipSuccess) ||
 	    hipMemset(bufs[2], 0xff, sizes[2]) != hipSuccess || hipMemset(bufs[3], 0, sizes[3]) != hipSuccess ||
@@ -296,6 +306,7 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	out->run_ms = (double)S[8] / ticks_per_ms;
 	out->drained = S[9];
 	out->drain_frees = S[10];
+	out->preempted = S[11];
 	out->ops_per_s = out->run_ms > 0 ? (double)(out->mallocs + out->frees) / (out->run_ms * 1e-3) : 0;
 	out->lat_p50_ns = hist_pct(hist, KMA_SV_NHIST, 0.50, tick_ns);
 	out->lat_p99_ns = hist_pct(hist, KMA_SV_NHIST, 0.99, tick_ns);
@@ -349,7 +360,7 @@ done:
 		hipEventDestroy(e1);
 	if (st)
 		hipStreamDestroy(st);
-	for (int i = 0; i < 13; i++)
+	for (int i = 0; i < 16; i++)
 		if (bufs[i])
 			hipFree(bufs[i]);
 	return rc;

@@ -20,7 +20,13 @@
  *     0 .. total-1, are spread over the whole grid: each thread finds its
  *     entry by binary search, checks the block's pattern, adds it to the
  *     job's result and frees the block with kma_wg_free.
- * A NULL from malloc stalls that job for the step (retried next step).
+ * A NULL from malloc stalls that job for the step (retried next step).  After
+ * a step with stalls, phase A preempts the most recently admitted live jobs
+ * until their blocks cover the shortfall (their blocks are freed in phase C,
+ * without adding to their results) and puts them on a requeue ring that is
+ * admitted before new arrivals; no job is admitted after a step with stalls.
+ * This keeps the loop from filling the heap with jobs that can then never
+ * grow (admission by a use watermark alone cannot know how long jobs grow).
  * Barriers release (write back L2) and acquire (invalidate) at agent scope,
  * so plain stores of one phase are visible to every XCD in the next.  Each
  * barrier wait has a 5 s watchdog that aborts the whole loop.
@@ -53,7 +59,13 @@ enum : uint32_t {
 	C_T0 = 96,
 	C_ABORT = 112,
 	C_LIVE = 128,
+	C_LTOP = 192,      /* phase A only: live-stack top, requeue head/tail, sequence */
+	C_RQ_HEAD = 193,
+	C_RQ_TAIL = 194,
+	C_SEQ = 195,
+	C_STEP_STALLS = 240, /* stalls of the current step (all work-groups) */
 };
+constexpr uint64_t PFLAG = 1ull << 63; /* done-list slot field: preempted, not completed */
 /* counters at KMA_SV_STATS + k */
 enum : uint32_t {
 	S_STEPS,
@@ -67,6 +79,7 @@ enum : uint32_t {
 	S_RUN_TICKS,
 	S_DRAINED,
 	S_DRAIN_FREES,
+	S_PREEMPT,
 	S_N
 };
 constexpr uint64_t MASK40 = (1ull << 40) - 1ull;
@@ -123,50 +136,108 @@ __device__ bool grid_sync(uint64_t *ctl, uint32_t nwg)
 	return s_ok != 0;
 }
 
+__device__ __forceinline__ bool lentry_live(const kma_serve_args &a, uint64_t e)
+{
+	uint32_t slot = (uint32_t)e;
+	return a.slot_job[slot] >= 0 && a.slot_seq[slot] == (uint32_t)(e >> 32);
+}
+
 /* Phase A, one lane. */
 __device__ void admit(const kma_serve_args &a, uint64_t t0, uint64_t pooled)
 {
-	using namespace kma_nat;
 	const kma_dheap *H = (const kma_dheap *)a.heap;
 	uint64_t *ctl = a.ctl;
+	uint64_t *S = ctl + KMA_SV_STATS;
 	uint64_t now = wall_clock64() - t0;
 	uint64_t nfree = ctl[C_NFREE], live = ctl[C_LIVE], next = ctl[C_NEXT];
+	uint64_t ltop = ctl[C_LTOP], rqh = ctl[C_RQ_HEAD], rqt = ctl[C_RQ_TAIL], seq = ctl[C_SEQ];
+	/* retire the previous step's entries: completions and preemptions */
 	uint64_t dc = ctl[C_DONE];
 	uint32_t ne = (uint32_t)(dc >> 40);
 	for (uint32_t e = 0; e < ne; e++) {
-		uint64_t slot = a.done[4u * e], job = a.done[4u * e + 1u];
-		a.job_done[job] = now + 1u;
-		a.slot_job[slot] = -1;
-		a.free_slots[nfree++] = (uint32_t)slot;
-		live--;
+		uint64_t sf = a.done[4u * e], job = a.done[4u * e + 1u];
+		uint32_t slot = (uint32_t)(sf & ~PFLAG);
+		if (sf & PFLAG) {
+			a.requeue[rqt++ % a.rq_cap] = job;
+		} else {
+			a.job_done[job] = now + 1u;
+			a.slot_job[slot] = -1;
+			S[S_DONE]++;
+			live--;
+		}
+		a.free_slots[nfree++] = slot;
 	}
-	ctl[C_DONE] = 0;
-	uint64_t *S = ctl + KMA_SV_STATS;
-	S[S_DONE] += ne;
+	dc = 0;
+	uint64_t stalls = ctl[C_STEP_STALLS];
+	ctl[C_STEP_STALLS] = 0;
 	uint64_t used = H->nsb > pooled ? H->nsb - pooled : 0u;
 	if (used > S[S_PEAK_USED])
 		S[S_PEAK_USED] = used;
-	bool stop = now >= a.duration_ticks || (next >= a.njobs && live == 0);
-	if (!stop) {
+	bool stop = now >= a.duration_ticks || (next >= a.njobs && live == 0 && rqh == rqt);
+	if (!stop && stalls) {
+		/* preempt the youngest live jobs until their blocks cover the
+		 * shortfall; their blocks go to this step's free list */
+		uint64_t need = stalls;
+		while (need && ltop && rqt - rqh < a.rq_cap) {
+			uint64_t e = a.lstack[--ltop];
+			if (!lentry_live(a, e))
+				continue;
+			uint32_t slot = (uint32_t)e;
+			uint64_t blocks = a.slot_steps[slot];
+			uint64_t k = dc >> 40;
+			a.done[4u * k] = slot | PFLAG;
+			a.done[4u * k + 1u] = (uint64_t)a.slot_job[slot];
+			a.done[4u * k + 2u] = dc & MASK40;
+			a.done[4u * k + 3u] = blocks;
+			dc += ((uint64_t)1 << 40) + blocks;
+			a.slot_job[slot] = -1;
+			live--;
+			S[S_PREEMPT]++;
+			need -= need < blocks ? need : blocks;
+		}
+	}
+	ctl[C_DONE] = dc;
+	if (!stop && !stalls) {
 		uint32_t k = 0;
-		while (next < a.njobs && a.arrival[next] <= now && nfree > 0 && k < a.max_admit &&
-		       used * 100u < (uint64_t)a.watermark_pct * H->nsb) {
+		while (nfree > 0 && k < a.max_admit && used * 100u < (uint64_t)a.watermark_pct * H->nsb) {
+			uint64_t job;
+			if (rqh != rqt) {
+				job = a.requeue[rqh++ % a.rq_cap];
+			} else if (next < a.njobs && a.arrival[next] <= now) {
+				job = next++;
+				a.job_admit[job] = now + 1u;
+			} else {
+				break;
+			}
 			uint32_t slot = a.free_slots[--nfree];
-			a.slot_job[slot] = (int64_t)next;
+			a.slot_job[slot] = (int64_t)job;
 			a.slot_steps[slot] = 0;
-			a.job_admit[next] = now + 1u;
-			next++;
+			a.slot_seq[slot] = (uint32_t)++seq;
+			if (ltop == a.lstack_cap) {
+				/* drop dead entries; at most max_live stay */
+				uint64_t w = 0;
+				for (uint64_t r = 0; r < ltop; r++)
+					if (lentry_live(a, a.lstack[r]))
+						a.lstack[w++] = a.lstack[r];
+				ltop = w;
+			}
+			a.lstack[ltop++] = (seq << 32) | slot;
 			live++;
 			k++;
 		}
 		S[S_ADMITTED] += k;
-		S[S_STEPS]++;
-	} else {
-		S[S_RUN_TICKS] = now;
 	}
+	if (!stop)
+		S[S_STEPS]++;
+	else
+		S[S_RUN_TICKS] = now;
 	ctl[C_NFREE] = nfree;
 	ctl[C_LIVE] = live;
 	ctl[C_NEXT] = next;
+	ctl[C_LTOP] = ltop;
+	ctl[C_RQ_HEAD] = rqh;
+	ctl[C_RQ_TAIL] = rqt;
+	ctl[C_SEQ] = seq;
 	ctl[C_STOP] = stop ? 1u : 0u;
 }
 
@@ -203,16 +274,24 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 		/* ---- B: grow every live job by one block ---- */
 		for (uint32_t sl = blockIdx.x; sl < nslices; sl += nwg) {
 			const uint32_t s = sl * SV_BLOCK + tid;
-			const int64_t job = a.slot_job[s];
+			const int64_t job0 = a.slot_job[s];
+			const uint32_t steps = job0 >= 0 ? a.slot_steps[s] : 0u;
+			/* a slot's block table holds max_len entries; a job never
+			 * outgrows its length <= max_len, so this only guards */
+			const bool over = job0 >= 0 && steps >= a.max_len;
+			c_bad += over;
+			const int64_t job = over ? -1 : job0;
 			const bool live = job >= 0;
-			const uint32_t steps = live ? a.slot_steps[s] : 0u;
 			const uint64_t t = wall_clock64();
 			char *p = (char *)kma_wg_malloc<SV_BLOCK>(H, live ? a.block_bytes : 0u, L);
 			const uint64_t dt = wall_clock64() - t;
 			const int nl = __syncthreads_count(live);
+			const int ns = __syncthreads_count(live && !p);
 			if (tid == 0 && nl)
 				atomicAdd((unsigned long long *)&a.lat_hist[dt < KMA_SV_NHIST ? dt : KMA_SV_NHIST - 1],
 					  (unsigned long long)nl);
+			if (tid == 0 && ns)
+				fadd(ctl + C_STEP_STALLS, (uint64_t)ns);
 			if (live) {
 				if (p) {
 					a.table[(uint64_t)s * a.max_len + steps] = (uint64_t)(uintptr_t)p;
@@ -250,12 +329,14 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 					else
 						hi = mid - 1u;
 				}
-				const uint64_t slot = a.done[4u * lo], job = a.done[4u * lo + 1u];
+				const uint64_t sf = a.done[4u * lo], job = a.done[4u * lo + 1u];
+				const uint64_t slot = sf & ~PFLAG;
 				const uint64_t k = x - a.done[4u * lo + 2u];
 				q = (void *)(uintptr_t)a.table[slot * a.max_len + k];
 				const uint64_t v = *reinterpret_cast<const uint64_t *>(q);
 				c_bad += v != pattern(pattern_id(a, job), k);
-				atomicAdd((unsigned long long *)&a.result[job], (unsigned long long)v);
+				if (!(sf & PFLAG))
+					atomicAdd((unsigned long long *)&a.result[job], (unsigned long long)v);
 				c_frees++;
 			}
 			kma_wg_free<SV_BLOCK>(H, q, L);
@@ -263,7 +344,8 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 		if (!grid_sync(ctl, nwg))
 			return;
 	}
-	/* ---- drain: free the blocks of jobs still live (not completions) ---- */
+	/* ---- drain: free the blocks of jobs still live (not completions); the
+	 * last phase A retired the previous step's entries and preempted none */
 	uint64_t c_drained = 0, c_dfrees = 0;
 	for (uint32_t sl = blockIdx.x; sl < nslices; sl += nwg) {
 		const uint32_t s = sl * SV_BLOCK + tid;

@@ -30,7 +30,7 @@ def check_stats(st):
     assert not st["aborted"]
     assert st["bad"] == 0
     assert st["mallocs"] == st["frees"] + st["drain_frees"], st
-    assert st["admitted"] == st["done"] + st["drained"], st
+    assert st["admitted"] == st["done"] + st["drained"] + st["preempted"], st
 
 
 def test_serve_mix_and_alone():
@@ -65,7 +65,8 @@ def test_serve_memory_pressure():
     with serve_heap(sblocks=16) as h:  # 16 MiB: 1,024 blocks
         st, res = kma_amd.serve_run(h, spec, nresults=1 << 20)
         check_stats(st)
-        assert st["admitted"] < st["jobs"], st  # the watermark held jobs back
+        assert st["admitted"] - st["preempted"] < st["jobs"], st  # admission held jobs back
+        assert st["stalls"] > 0 and st["preempted"] > 0, st  # the heap ran out and jobs were preempted
         done = np.nonzero(res)[0].astype(np.uint64)
         assert len(done) > 0
         assert np.array_equal(res[done.astype(np.int64)], expected(spec, done))
