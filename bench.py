@@ -34,6 +34,21 @@ SB_LOG2 = 14
 SBLOCKS = 65536  # 1 GiB
 SEED = 1
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30  # the copy probe: reads and writes 1 GiB
+# Counter-measured HBM traffic of the batched launch, per round, from the
+# FETCH_SIZE and WRITE_SIZE profiles (scaled by the copy probe's reading).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_full_occupancy_batched.md")
+
+
+def traffic_per_round():
+    """Bytes per round from the committed counter profile, or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_round:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 # Algorithmic bytes per heap op: one 4-B bitmap word read-modify-write per
 # malloc or free (8 B), plus the workload payload per work-item-round
 # (pattern write + mean 32 hold writes + read-back, 4 B each) split over
@@ -172,6 +187,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--only", choices=["plain", "fused", "batched"],
+                    help="run only this form of the full-occupancy workload, then the copy "
+                         "probe (the command the counter profiles are taken with)")
     ap.add_argument("--serve-ms", type=float, default=10000.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -185,6 +203,15 @@ def main():
         raise SystemExit("no GPU: " + kma_amd.last_error())
 
     K, W = args.steps, args.warmup
+    if args.only:
+        form = {"plain": kma_amd.FORM_PLAIN, "fused": kma_amd.FORM_FUSED, "batched": kma_amd.FORM_BATCHED}[args.only]
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS) as h:
+            st = h.run(kma_amd.WL_LOGUNI, NWI, K, form=form, block=BLOCK, seed=SEED, warmup=W)
+        gbps, cms = kma_amd.copy_probe(COPY_BYTES, warmup=1, reps=1)
+        print(json.dumps({"form": args.only, "rounds": K, "launches": st["launches"],
+                          "kernel_ms_avg": st["kernel_ms_avg"], "ops": st["ops"], "bad": st["bad"],
+                          "copy_bytes": COPY_BYTES, "copy_gbps": gbps, "copy_ms": cms}), flush=True)
+        return
     res = {}
     for form, name in ((kma_amd.FORM_PLAIN, "plain"), (kma_amd.FORM_FUSED, "fused"),
                        (kma_amd.FORM_BATCHED, "batched")):
@@ -256,7 +283,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (traffic_per_round() * K) if traffic_per_round() else None,
         },
         "atomic_roofline": {
             "peak_distinct_atomics_per_s": peak["distinct"],

@@ -86,6 +86,12 @@ int kma_trace_serial(kma_heap *h, const int64_t *ops, uint64_t n, uint64_t *out,
 int kma_atomic_peak(uint64_t bytes, uint64_t nthreads, uint32_t iters, int pattern,
 		    uint32_t warmup, uint32_t reps, double *ops_per_s, double *ms);
 
+/* Streaming copy of `bytes` bytes between two device buffers (k_copy):
+ * *gbps = (read + written bytes) per second, *ms = average launch time.
+ * The HBM bandwidth reference, and the known-size kernel that scales the
+ * FETCH_SIZE / WRITE_SIZE counters. */
+int kma_copy_probe(uint64_t bytes, uint32_t warmup, uint32_t reps, double *gbps, double *ms);
+
 /* Self-check of the wavefront bitmap claim on a native heap (test hook):
  * case i claims got[i] blocks of class cls[i] from superblock i, whose
  * bitmap (bm_words words) is first set to bm_in[i]; returns the block

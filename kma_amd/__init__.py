@@ -27,7 +27,7 @@ EXPORTS = [
     "kma_create", "kma_create_ex", "kma_destroy", "kma_device_heap",
     "kma_heap_bytes", "kma_get_config", "kma_snapshot", "kma_reset",
     "kma_last_error", "kma_device_ok", "kma_run_workload", "kma_trace_serial",
-    "kma_trim", "kma_atomic_peak", "kma_check_wave_claim",
+    "kma_trim", "kma_atomic_peak", "kma_check_wave_claim", "kma_copy_probe",
     "kma_stream_create", "kma_stream_destroy", "kma_stream_get_info",
     "kma_stream_trace", "kma_stream_results", "kma_stream_times", "kma_stream_run",
     "kma_stream_heap_config", "kma_stream_autotune",
@@ -227,6 +227,7 @@ def lib():
         L.kma_check_wave_claim.argtypes = [vp, u32, vp, vp, vp, vp, vp, vp, vp]
         L.kma_atomic_peak.argtypes = [u64, u64, u32, ctypes.c_int, u32, u32,
                                       P(ctypes.c_double), P(ctypes.c_double)]
+        L.kma_copy_probe.argtypes = [u64, u32, u32, P(ctypes.c_double), P(ctypes.c_double)]
         L.kma_stream_create.argtypes = [P(StreamSpec)]
         L.kma_stream_create.restype = vp
         L.kma_stream_destroy.argtypes = [vp]
@@ -694,3 +695,11 @@ def serve_pattern(job, k):
         z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         return z ^ (z >> np.uint64(31))
+
+
+def copy_probe(nbytes, warmup=2, reps=5):
+    """(GB/s read+written, ms per launch) of a streaming device copy."""
+    g = ctypes.c_double()
+    ms = ctypes.c_double()
+    _check(lib().kma_copy_probe(nbytes, warmup, reps, ctypes.byref(g), ctypes.byref(ms)), "kma_copy_probe")
+    return g.value, ms.value

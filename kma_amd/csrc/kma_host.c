@@ -549,3 +549,55 @@ out:
 			hipFree(d[k]);
 	return rc;
 }
+
+int kma_copy_probe(uint64_t bytes, uint32_t warmup, uint32_t reps, double *gbps, double *ms_out)
+{
+	if (bytes < 4096 || (bytes & 15) || reps == 0) {
+		set_err("bad arguments");
+		return -1;
+	}
+	if (kma_device_ok() != 0)
+		return -1;
+	void *a = NULL, *b = NULL;
+	hipStream_t s = NULL;
+	hipEvent_t e0 = NULL, e1 = NULL;
+	int rc = -1;
+	if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+	    hipMemset(a, 1, bytes) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
+	    hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+		set_err("copy probe setup failed");
+		goto done;
+	}
+	for (uint32_t i = 0; i < warmup; i++)
+		if ((rc = kmad_copy(a, b, bytes, s)) != 0)
+			goto done;
+	hipEventRecord(e0, s);
+	for (uint32_t i = 0; i < reps; i++)
+		if ((rc = kmad_copy(a, b, bytes, s)) != 0)
+			goto done;
+	hipEventRecord(e1, s);
+	if (hipEventSynchronize(e1) != hipSuccess) {
+		set_err("copy kernel failed");
+		rc = -1;
+		goto done;
+	}
+	float ms = 0;
+	hipEventElapsedTime(&ms, e0, e1);
+	if (gbps)
+		*gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+	if (ms_out)
+		*ms_out = ms / reps;
+	rc = 0;
+done:
+	if (e0)
+		hipEventDestroy(e0);
+	if (e1)
+		hipEventDestroy(e1);
+	if (s)
+		hipStreamDestroy(s);
+	if (a)
+		hipFree(a);
+	if (b)
+		hipFree(b);
+	return rc;
+}

@@ -243,6 +243,26 @@ __global__ void __launch_bounds__(256) k_atomic_peak(uint32_t *buf, uint64_t nwo
 		buf[0] = acc; /* keep the results live */
 }
 
+/* Streaming copy of nvec 16-byte vectors (grid-stride): the reference
+ * kernel for HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE
+ * counters (it reads and writes exactly nvec * 16 bytes). */
+__global__ void __launch_bounds__(256) k_copy(const uint4 *src, uint4 *dst, uint64_t nvec)
+{
+	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (; i < nvec; i += stride)
+		dst[i] = src[i];
+}
+
+extern "C" int kmad_copy(const void *src, void *dst, uint64_t bytes, void *stream)
+{
+	uint64_t nvec = bytes / 16;
+	hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst,
+			   nvec);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
 extern "C" int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, uint32_t iters, int pattern,
 				void *stream)
 {

@@ -41,6 +41,7 @@ int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t *cls, cons
 
 int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, uint32_t iters,
 		     int pattern, void *stream);
+int kmad_copy(const void *src, void *dst, uint64_t bytes, void *stream);
 
 /* Batched request streams (kma_stream.hip).  ctl is a device array of
  * KMA_CTL_WORDS u64: epoch, chunk counter and counters. */

@@ -25,6 +25,27 @@ __device__ __forceinline__ uint64_t kma_mix(uint64_t seed, uint64_t gid, uint64_
 	return z ^ (z >> 31);
 }
 
+/* Workload payload accesses are ordinary cached stores and loads.  The
+ * empty asm with a memory clobber after each store keeps the compiler from
+ * merging or dropping any of them (every hold iteration's write is issued),
+ * and before the read-back it forces a real load.  (A volatile pointer would
+ * make each access a system-scope write-through with a wait after it.) */
+__device__ __forceinline__ void wl_store(uint32_t *p, uint32_t v)
+{
+	*p = v;
+	__asm__ __volatile__("" ::: "memory");
+}
+__device__ __forceinline__ uint32_t wl_load(const uint32_t *p)
+{
+	__asm__ __volatile__("" ::: "memory");
+	return *p;
+}
+__device__ __forceinline__ void wl_store64(uint64_t *p, uint64_t v)
+{
+	*p = v;
+	__asm__ __volatile__("" ::: "memory");
+}
+
 __device__ __forceinline__ void wave_add_stats(unsigned long long *stats, uint64_t a, uint64_t b, uint64_t c,
 					       uint64_t d)
 {
@@ -119,7 +140,7 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 			}
 			ph.start();
 			uint64_t t0 = a.lat ? wall_clock64() : 0;
-			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, size);
+			uint32_t *p = (uint32_t *)kma_malloc(heap, size);
 			if (a.lat)
 				a.lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
 			ph.mark(0);
@@ -131,16 +152,16 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 			}
 			uint32_t pat = (uint32_t)(gid << 8) ^ r;
 			if (mode == KMA_WL_FIXED) {
-				p[0] = (uint32_t)gid;
-				p[1] = r;
-				if (p[0] != (uint32_t)gid || p[1] != r)
+				wl_store(p, (uint32_t)gid);
+				wl_store(p + 1, r);
+				if (wl_load(p) != (uint32_t)gid || wl_load(p + 1) != r)
 					bad++;
 			} else {
 				uint32_t words = size >> 2;
-				p[0] = pat;
+				wl_store(p, pat);
 				for (uint32_t k = 0; k < hold; k++)
-					p[1 + k % (words - 1)] = pat + k + 1;
-				if (p[0] != pat)
+					wl_store(p + 1 + k % (words - 1), pat + k + 1);
+				if (wl_load(p) != pat)
 					bad++;
 			}
 			ph.mark(1);
@@ -148,7 +169,7 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 			ph.mark(2);
 			ops++;
 		} else if (mode == KMA_WL_REF_TEST) {
-			volatile uint64_t *p = (volatile uint64_t *)kma_malloc(heap, 8);
+			uint64_t *p = (uint64_t *)kma_malloc(heap, 8);
 			mallocs++;
 			ops++;
 			if (!p) {
@@ -156,7 +177,7 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 				alive = false;
 				continue;
 			}
-			p[0] = gid;
+			wl_store64(p, gid);
 			kma_free(heap, (void *)p);
 			ops++;
 		} else {
@@ -165,7 +186,7 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 				amount = (gid & 1) ? 8 : 16;
 			else
 				amount = 4u << ((gid + r) % 5);
-			volatile uint32_t *p = (volatile uint32_t *)kma_malloc(heap, amount);
+			uint32_t *p = (uint32_t *)kma_malloc(heap, amount);
 			mallocs++;
 			ops++;
 			if (!p) {
@@ -173,9 +194,9 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 				alive = false;
 				continue;
 			}
-			p[0] = mode == KMA_WL_LOWVAR ? (uint32_t)gid : 4919u;
+			wl_store(p, mode == KMA_WL_LOWVAR ? (uint32_t)gid : 4919u);
 			if (gid == 0) {
-				volatile uint32_t *q = (volatile uint32_t *)kma_malloc(heap, 4000);
+				uint32_t *q = (uint32_t *)kma_malloc(heap, 4000);
 				mallocs++;
 				ops++;
 				if (!q) {
@@ -183,7 +204,7 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 					alive = false;
 					continue;
 				}
-				q[8] = 4919;
+				wl_store(q + 8, 4919u);
 				kma_free(heap, (void *)q);
 				ops++;
 			}
@@ -217,13 +238,13 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
 			}
 		}
-		volatile uint32_t *p;
+		uint32_t *p;
 		ph.start();
 		uint64_t t0 = a.lat ? wall_clock64() : 0;
 		if (variant & 1u)
-			p = (volatile uint32_t *)kma_malloc(H, size);
+			p = (uint32_t *)kma_malloc(H, size);
 		else
-			p = (volatile uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, (variant >> 2) & 1u);
+			p = (uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, (variant >> 2) & 1u);
 		if (a.lat && active)
 			a.lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
 		ph.mark(0);
@@ -235,16 +256,16 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 			} else {
 				uint32_t pat = (uint32_t)(gid << 8) ^ r;
 				if (mode == KMA_WL_FIXED) {
-					p[0] = (uint32_t)gid;
-					p[1] = r;
-					if (p[0] != (uint32_t)gid || p[1] != r)
+					wl_store(p, (uint32_t)gid);
+					wl_store(p + 1, r);
+					if (wl_load(p) != (uint32_t)gid || wl_load(p + 1) != r)
 						bad++;
 				} else {
 					uint32_t words = size >> 2;
-					p[0] = pat;
+					wl_store(p, pat);
 					for (uint32_t k = 0; k < hold; k++)
-						p[1 + k % (words - 1)] = pat + k + 1;
-					if (p[0] != pat)
+						wl_store(p + 1 + k % (words - 1), pat + k + 1);
+					if (wl_load(p) != pat)
 						bad++;
 				}
 				ops++;
