@@ -9,9 +9,10 @@
  * stages the batch in LDS so that the device-wide atomics are issued per
  * size class and per bitmap word instead of per request:
  *   malloc: LDS class histogram -> one reservation (atomicAdd on the
- *           superblock state word) per class; one wavefront per class reads
- *           the superblock bitmap coalesced (lane i <- word i), prefix-sums the
- *           free bits and claims them with one atomicOr per word.
+ *           superblock state word) per class; a 16-lane group per class
+ *           (all classes of the batch at once) reads the superblock bitmap
+ *           coalesced (lane i <- word i), prefix-sums the free bits and claims
+ *           them with one atomicOr per word.
  *   free:   an LDS hash keyed by (superblock, bitmap word) ORs the bits of all
  *           requests that share a word; one atomicAnd per word and one
  *           atomicSub on the state word per entry.
@@ -35,19 +36,6 @@ struct kma_wg_lds {
 
 namespace kma_nat {
 
-/* Exclusive prefix sum over a full wavefront (all 64 lanes active). */
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total)
-{
-	uint32_t x = v;
-	for (int o = 1; o < 64; o <<= 1) {
-		uint32_t y = __shfl_up(x, o);
-		if ((int)lane() >= o)
-			x += y;
-	}
-	*total = __shfl(x, 63);
-	return x - v;
-}
-
 /* The lowest `k` set bits of `m`. */
 __device__ __forceinline__ uint32_t low_bits(uint32_t m, uint32_t k)
 {
@@ -60,56 +48,91 @@ __device__ __forceinline__ uint32_t low_bits(uint32_t m, uint32_t k)
 	return r;
 }
 
-/* One full wavefront claims `got` free bits of superblock sb (class c),
- * starting its scan at the word holding bit `base`, and writes the block
- * offsets (from the heap base) into res[order[0..got-1]]. */
-__device__ inline uint32_t wave_claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t base, uint32_t got,
-				      const uint16_t *order, uint64_t *res)
+/* ---- 16-lane groups: one size class per group, four per wavefront ----
+ * Every lane of the wavefront executes these (the loops are
+ * wavefront-uniform); a group with nothing to do passes dummies. */
+#define KMA_CLAIM_G 16u
+
+__device__ __forceinline__ uint32_t grp_lane() { return lane() & (KMA_CLAIM_G - 1u); }
+
+__device__ __forceinline__ uint32_t group_excl_scan(uint32_t v, uint32_t *total)
 {
-	uint32_t cap = H->class_cap[c];
+	uint32_t x = v, gl = grp_lane();
+	for (uint32_t o = 1; o < KMA_CLAIM_G; o <<= 1) {
+		uint32_t y = __shfl_up(x, o, (int)KMA_CLAIM_G);
+		if (gl >= o)
+			x += y;
+	}
+	*total = __shfl(x, (int)KMA_CLAIM_G - 1, (int)KMA_CLAIM_G);
+	return x - v;
+}
+
+__device__ __forceinline__ uint32_t group_bcast(uint32_t v)
+{
+	return __shfl(v, 0, (int)KMA_CLAIM_G);
+}
+
+/* Each group with want set claims `got` free bits of its superblock sb
+ * (class c), scanning from the word that holds bit `base`, 16 words per
+ * pass, and writes block offsets (from the heap base) into
+ * res[order[0..done-1]]; returns the group's `done`.  After a full pass
+ * over the bitmap without progress the words are re-read with RMWs. */
+__device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t sb, uint32_t c, uint32_t base,
+				       uint32_t got, const uint16_t *order, uint64_t *res)
+{
+	const uint32_t gl = grp_lane();
+	uint32_t cap = want ? H->class_cap[c] : 32u;
 	uint32_t W = (cap + 31u) >> 5;
 	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
-	uint32_t *bm = bitmap(H, sb);
-	uint64_t d0 = H->off_data + ((uint64_t)sb << H->sb_log2);
-	uint32_t bsize = H->class_size[c];
-	uint32_t done = 0;
-	uint32_t w0 = (base >> 5) % W;
-	bool fresh = false; /* after a pass without progress, re-read with RMWs */
+	uint32_t *bm = want ? bitmap(H, sb) : nullptr;
+	uint64_t d0 = want ? H->off_data + ((uint64_t)sb << H->sb_log2) : 0u;
+	uint32_t bsize = want ? H->class_size[c] : 0u;
+	uint32_t w0 = want ? (base >> 5) % W : 0u;
+	uint32_t done = 0, chunk = 0, before = 0;
+	bool fresh = false, need = want && got > 0;
 	waiter wt;
-	while (done < got && !wt.expired()) {
-		uint32_t before = done;
-		for (uint32_t chunk = 0; chunk < W && done < got; chunk += 64) {
-			uint32_t k = chunk + lane();
-			bool has = k < W;
-			uint32_t wi = has ? (w0 + k) % W : 0;
-			uint32_t freeb = 0;
-			if (has) {
-				freeb = ~(fresh ? f_or(bm + wi, 0u) : ld(bm + wi));
-				if (wi == W - 1)
-					freeb &= tail;
-			}
-			uint32_t tot;
-			uint32_t pre = wave_excl_scan(__popc(freeb), &tot);
-			uint32_t need = got - done;
-			uint32_t take = min((uint32_t)__popc(freeb), sat_sub(need, pre));
-			uint32_t mask = low_bits(freeb, take);
-			uint32_t won = 0;
-			if (mask)
-				won = mask & ~f_or(bm + wi, mask);
-			uint32_t wtot;
-			uint32_t wpre = wave_excl_scan(__popc(won), &wtot);
-			uint32_t r = done + wpre;
-			while (won) {
-				uint32_t b = __ffs(won) - 1u;
-				won &= won - 1u;
-				res[order[r++]] = d0 + (uint64_t)(wi * 32u + b) * bsize;
-			}
-			done += wtot;
+	while (__ballot(need)) {
+		uint32_t k = chunk + gl;
+		bool has = need && k < W;
+		uint32_t wi = has ? (w0 + k) % W : 0u;
+		uint32_t freeb = 0;
+		if (has) {
+			freeb = ~(fresh ? f_or(bm + wi, 0u) : ld(bm + wi));
+			if (wi == W - 1u)
+				freeb &= tail;
 		}
-		fresh = done == before;
+		uint32_t tot;
+		uint32_t pre = group_excl_scan(__popc(freeb), &tot);
+		uint32_t take = need ? min((uint32_t)__popc(freeb), sat_sub(got - done, pre)) : 0u;
+		uint32_t mask = low_bits(freeb, take);
+		uint32_t won = 0;
+		if (mask)
+			won = mask & ~f_or(bm + wi, mask);
+		uint32_t wtot;
+		uint32_t wpre = group_excl_scan(__popc(won), &wtot);
+		uint32_t r = done + wpre;
+		while (won) {
+			uint32_t b = __ffs(won) - 1u;
+			won &= won - 1u;
+			res[order[r++]] = d0 + (uint64_t)(wi * 32u + b) * bsize;
+		}
+		const bool exp = group_bcast(need && wt.expired() ? 1u : 0u) != 0;
+		if (need) {
+			done += wtot;
+			chunk += KMA_CLAIM_G;
+			if (chunk >= W) {
+				/* a full pass: re-read with RMWs if it made no progress */
+				chunk = 0;
+				fresh = done == before;
+				before = done;
+			}
+			if (done >= got || exp) {
+				if (done < got && gl == 0)
+					bump_stat(H, KMA_STAT_WAIT_WGCLAIM);
+				need = false;
+			}
+		}
 	}
-	if (done < got && lane() == 0)
-		bump_stat(H, KMA_STAT_WAIT_WGCLAIM);
 	return done;
 }
 
@@ -152,46 +175,55 @@ __device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOC
 	if (c >= 0)
 		L.order[L.cbase[c] + q] = (uint16_t)tid;
 	__syncthreads();
-	/* one wavefront per class: reserve a segment (lane 0), claim it (all
-	 * lanes), until the class is served or the heap cannot serve it */
-	for (uint32_t k = w; k < ncls; k += nw) {
-		uint32_t n = L.cnt[k];
-		if (n == 0)
-			continue;
-		const uint16_t *ord = &L.order[L.cbase[k]];
+	/* one 16-lane group per class, all classes at once: the group's first
+	 * lane reserves a segment, the group claims it; repeat until the class is
+	 * served or the heap cannot serve it (loops are wavefront-uniform) */
+	const uint32_t G = KMA_CLAIM_G, ngroups = nw * (64u / G), g = tid / G, gl = tid % G;
+	const uint32_t iters = (ncls + ngroups - 1u) / ngroups;
+	for (uint32_t it = 0; it < iters; it++) {
+		const uint32_t k = g + it * ngroups;
+		const uint32_t n = k < ncls ? L.cnt[k] : 0u;
+		const uint16_t *ord = n ? &L.order[L.cbase[k]] : L.order;
 		uint32_t r0 = 0;
-		while (r0 < n) {
+		bool pend = n > 0;
+		while (__ballot(pend)) {
 			resv r = {0, 0, 0, 0};
-			if (lane() == 0)
+			if (pend && gl == 0)
 				r = reserve(H, x, ctx, k, n - r0);
-			uint32_t ok = rdlane(r.ok, 0);
-			if (!ok)
-				break;
-			uint32_t sb = rdlane(r.sb, 0), rb = rdlane(r.base, 0), got = rdlane(r.got, 0);
+			const uint32_t ok = group_bcast(r.ok), sb = group_bcast(r.sb);
+			const uint32_t rb = group_bcast(r.base), got = group_bcast(r.got);
+			const bool want = pend && ok;
+			if (pend && !ok)
+				pend = false;
 			uint32_t done;
 			if (per_lane_claim) {
-				/* one claim per block: lane j takes ranks j, j+64, ... */
+				/* one claim per block: group lane j takes ranks j, j+16, ... */
 				uint32_t mine = 0;
-				for (uint32_t q = lane(); q < got; q += 64) {
-					char *p = claim(H, sb, k, rb + q);
-					L.res[ord[r0 + q]] = p ? (uint64_t)(p - base(H)) : 0;
-					mine += p ? 1u : 0u;
-				}
+				if (want)
+					for (uint32_t q = gl; q < got; q += G) {
+						char *p = claim(H, sb, k, rb + q);
+						L.res[ord[r0 + q]] = p ? (uint64_t)(p - base(H)) : 0;
+						mine += p ? 1u : 0u;
+					}
 				uint32_t tot;
-				wave_excl_scan(mine, &tot);
+				group_excl_scan(mine, &tot);
 				done = tot;
 			} else {
-				done = wave_claim(H, sb, k, rb, got, ord + r0, L.res);
+				done = group_claim(H, want, sb, k, rb, got, ord + r0, L.res);
 			}
-			r0 += done;
-			if (done < got) {
-				/* give back what could not be claimed */
-				if (lane() == 0)
-					undo(H, sb, got - done, x, ctx);
-				break;
+			if (want) {
+				r0 += done;
+				if (done < got) {
+					/* give back what could not be claimed */
+					if (gl == 0)
+						undo(H, sb, got - done, x, ctx);
+					pend = false;
+				} else if (r0 >= n) {
+					pend = false;
+				}
 			}
 		}
-		for (uint32_t r = r0 + lane(); r < n; r += 64)
+		for (uint32_t r = r0 + gl; r < n; r += G)
 			L.res[ord[r]] = 0;
 	}
 	__syncthreads();

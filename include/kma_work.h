@@ -92,8 +92,9 @@ int kma_atomic_peak(uint64_t bytes, uint64_t nthreads, uint32_t iters, int patte
  * FETCH_SIZE / WRITE_SIZE counters. */
 int kma_copy_probe(uint64_t bytes, uint32_t warmup, uint32_t reps, double *gbps, double *ms);
 
-/* Self-check of the wavefront bitmap claim on a native heap (test hook):
- * case i claims got[i] blocks of class cls[i] from superblock i, whose
+/* Self-check of the bitmap claim of the work-group collective malloc (one
+ * 16-lane group per case, four cases per wavefront) on a native heap (test
+ * hook): case i claims got[i] blocks of class cls[i] from superblock i, whose
  * bitmap (bm_words words) is first set to bm_in[i]; returns the block
  * offsets in res_out[i*64 ..], the final bitmap in bm_out and the number
  * claimed in done_out[i].  Destroys the heap's contents. */

@@ -300,28 +300,41 @@ extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int m
 }
 
 /* ------------------------------------------------------------------ */
-/* wave_claim check: one wavefront per case, superblock `case` of a     */
-/* native heap preloaded with a bitmap; results back to the host        */
+/* bitmap claim check: four cases per wavefront, one per 16-lane group  */
+/* (the claim of the work-group collective malloc); case i uses          */
+/* superblock i of a native heap preloaded with a bitmap                 */
 /* ------------------------------------------------------------------ */
-__global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, const uint32_t *cls, const uint32_t *base,
-							 const uint32_t *got, const uint32_t *bm_in,
-							 uint64_t *res_out, uint32_t *bm_out, uint32_t *done_out)
+__global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, uint32_t ncases, const uint32_t *cls,
+							 const uint32_t *base, const uint32_t *got,
+							 const uint32_t *bm_in, uint64_t *res_out, uint32_t *bm_out,
+							 uint32_t *done_out)
 {
-	__shared__ uint16_t order[64];
-	__shared__ uint64_t res[64];
-	uint32_t cs = blockIdx.x, l = threadIdx.x;
-	uint32_t *bm = kma_nat::bitmap(H, cs);
-	for (uint32_t w = l; w < H->bm_words; w += 64)
-		bm[w] = bm_in[cs * H->bm_words + w];
-	order[l] = (uint16_t)l;
-	res[l] = 0;
+	__shared__ uint16_t order[4][64];
+	__shared__ uint64_t res[4][64];
+	const uint32_t l = threadIdx.x, g = l / KMA_CLAIM_G;
+	for (uint32_t q = 0; q < 4; q++) {
+		uint32_t cs = blockIdx.x * 4u + q;
+		if (cs < ncases)
+			for (uint32_t w = l; w < H->bm_words; w += 64)
+				kma_nat::bitmap(H, cs)[w] = bm_in[cs * H->bm_words + w];
+		order[q][l] = (uint16_t)l;
+		res[q][l] = 0;
+	}
 	__syncthreads();
-	uint32_t done = kma_nat::wave_claim(H, cs, cls[cs], base[cs], got[cs], order, res);
+	const uint32_t cs = blockIdx.x * 4u + g;
+	const bool want = cs < ncases;
+	uint32_t done = kma_nat::group_claim(H, want, cs, want ? cls[cs] : 0u, want ? base[cs] : 0u,
+					     want ? got[cs] : 0u, order[g], res[g]);
 	__syncthreads();
-	res_out[cs * 64 + l] = res[l];
-	for (uint32_t w = l; w < H->bm_words; w += 64)
-		bm_out[cs * H->bm_words + w] = kma_nat::ld(bm + w);
-	if (l == 0)
+	for (uint32_t q = 0; q < 4; q++) {
+		uint32_t c2 = blockIdx.x * 4u + q;
+		if (c2 >= ncases)
+			continue;
+		res_out[c2 * 64 + l] = res[q][l];
+		for (uint32_t w = l; w < H->bm_words; w += 64)
+			bm_out[c2 * H->bm_words + w] = kma_nat::ld(kma_nat::bitmap(H, c2) + w);
+	}
+	if (want && kma_nat::grp_lane() == 0)
 		done_out[cs] = done;
 }
 
@@ -329,8 +342,8 @@ extern "C" int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t
 				     const uint32_t *got, const uint32_t *bm_in, uint64_t *res_out, uint32_t *bm_out,
 				     uint32_t *done_out)
 {
-	hipLaunchKernelGGL(k_check_wave_claim, dim3(ncases), dim3(64), 0, 0, (kma_dheap *)heap, cls, base, got, bm_in,
-			   res_out, bm_out, done_out);
+	hipLaunchKernelGGL(k_check_wave_claim, dim3((ncases + 3u) / 4u), dim3(64), 0, 0, (kma_dheap *)heap, ncases, cls,
+			   base, got, bm_in, res_out, bm_out, done_out);
 	KMA_CHECK(hipGetLastError());
 	KMA_CHECK(hipDeviceSynchronize());
 	return 0;

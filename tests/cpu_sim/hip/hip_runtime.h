@@ -71,18 +71,19 @@ static inline T sim_exchange(T v, uint32_t src)
 	return r;
 }
 
+/* width splits the wavefront into segments of `width` lanes, as on the
+ * device: sources are taken within the caller's segment */
 template <typename T>
 static inline T __shfl(T v, int src, int width = 64)
 {
-	(void)width;
-	return sim_exchange(v, (uint32_t)src);
+	uint32_t l = __lane_id(), w = (uint32_t)width;
+	return sim_exchange(v, (l & ~(w - 1u)) + ((uint32_t)src & (w - 1u)));
 }
 template <typename T>
 static inline T __shfl_up(T v, unsigned int d, int width = 64)
 {
-	(void)width;
-	uint32_t l = __lane_id();
-	return sim_exchange(v, l >= d ? l - d : l);
+	uint32_t l = __lane_id(), w = (uint32_t)width;
+	return sim_exchange(v, (l & (w - 1u)) >= d ? l - d : l);
 }
 template <typename T>
 static inline T __shfl_xor(T v, int m, int width = 64)
