@@ -39,10 +39,10 @@
  *     the XCC id of the calling wavefront), so a block is normally reused on
  *     the XCD whose L2 holds its lines;
  *   - a free from another XCD first writes back its own L2 (release);
- *   - the last free of a superblock on its home XCD writes back the L2 and
- *     returns the superblock to a "clean" pool shard any XCD may take from;
- *     if the last free came from another XCD, the superblock goes to its home
- *     XCD's "dirty" shard, which only the home XCD pops from;
+ *   - an emptied superblock goes to one of its home XCD's "dirty" pool
+ *     shards, which only the home XCD pops from, with no write-back; other
+ *     XCDs take from "clean" shards, which a home XCD refills in batches of
+ *     up to 64 superblocks behind ONE L2 write-back, when asked (migration);
  *   - page frees always write back (pages are not homed).
  * A local free only waits for its own stores (s_waitcnt) before it clears
  * the block's bit.
@@ -104,11 +104,13 @@ typedef struct kma_dheap {
 #define KMA_STAT_WAIT_WGCLAIM 5 /* a wavefront bitmap claim hit the watchdog */
 #define KMA_STAT_WAIT_POOL 6    /* a pool push/pop hit the watchdog */
 #define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
-#define KMA_NSTATS 8
-/* The stats block is 256 B of counters followed by one 128-B line per XCD
- * holding that XCD's shard mask: bit k set when shard x*S + k may hold a
- * superblock (see pool_push / pool_pop_shard). */
-#define KMA_STATS_BYTES (256u + 128u * 8u)
+#define KMA_STAT_MIGRATED 8     /* superblocks moved from dirty to clean shards in a kernel */
+#define KMA_NSTATS 9
+/* The stats block is 256 B of counters, one 128-B line per XCD holding that
+ * XCD's shard mask (bit k set when shard x*S + k may hold a superblock, see
+ * pool_push / pool_pop_shard) and one 128-B line holding the clean-request
+ * word (bit x: another XCD asks XCD x to migrate dirty superblocks). */
+#define KMA_STATS_BYTES (256u + 128u * 9u)
 /* Page runs of k pages use an aligned window of 2^l >= k pages inside one
  * 64-page bitmap word; levels l = 0..6. */
 #define KMA_PAGE_LEVELS 7u
@@ -247,6 +249,7 @@ __device__ __forceinline__ uint8_t *plen(const kma_dheap *H) { return (uint8_t *
 __device__ __forceinline__ char *sbdata(const kma_dheap *H, uint64_t sb) { return base(H) + H->off_data + (sb << H->sb_log2); }
 __device__ __forceinline__ uint64_t *stats(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats); }
 __device__ __forceinline__ uint64_t *pmask(const kma_dheap *H, uint32_t x) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * x); }
+__device__ __forceinline__ uint64_t *creq(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * KMA_NXCD); }
 __device__ __forceinline__ uint64_t psum_words(const kma_dheap *H) { return (((H->npages + 63) >> 6) + 63) >> 6; }
 __device__ __forceinline__ uint64_t *psum(const kma_dheap *H, uint32_t l) { return (uint64_t *)(base(H) + H->off_psum) + l * psum_words(H); }
 
@@ -289,6 +292,15 @@ __device__ __forceinline__ uint32_t wg_ctx_of(const kma_dheap *H, uint32_t x)
 /* ---- superblock pool: sharded Treiber stacks, 64-bit {tag:32, top:32} ----
  * XCD x owns shards [x*S, x*S + S), S = npool/8 <= 64: the first D = S/2
  * are its dirty shards (only x pops them), the other S - D are clean.
+ * A superblock emptied on its home XCD x goes to a dirty shard of x with no
+ * cache maintenance: x's L2 may still hold dirty lines of its blocks, which
+ * must not be written back over another XCD's later use.  Another XCD gets
+ * such superblocks only through a migration run on x (migrate_dirty): pop a
+ * batch, write back x's L2 once, push the batch onto a clean shard.  An XCD
+ * that finds no clean superblock anywhere sets the bits of the XCDs holding
+ * dirty ones in the clean-request word; those XCDs migrate when they next
+ * pop a superblock, and kma_trim (after a kernel boundary) empties every
+ * dirty shard.
  * pmask(x) bit k is a hint that shard x*S + k is non-empty, updated only
  * when a shard goes empty <-> non-empty:
  *   - a push that found the shard empty sets the bit after its CAS;
@@ -324,18 +336,21 @@ __device__ inline void shard_emptied(const kma_dheap *H, uint32_t shard)
 		f_or(mp, bit);
 }
 
-/* Push never gives up: a failed CAS means another push or pop succeeded,
- * so the pool as a whole progresses; back-off spreads the retries. */
-__device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb)
+/* Push the chain first -> ... -> last (n superblocks linked through pnext)
+ * onto a shard.  A push never gives up: a failed CAS means another push or
+ * pop succeeded, so the pool as a whole progresses; back-off spreads the
+ * retries. */
+__device__ inline void pool_push_chain(const kma_dheap *H, uint32_t shard, uint32_t first, uint32_t last,
+				       uint32_t n)
 {
 	uint64_t *hp = pool(H, shard);
 	uint64_t h = ld(hp);
 	uint32_t bo = 1;
 	for (;;) {
 		/* the link must be performed before the head CAS publishes it */
-		xchg(pnext(H, sb), (uint32_t)h);
+		xchg(pnext(H, last), (uint32_t)h);
 		order_prior();
-		uint64_t nh = (((h >> 32) + 1ull) << 32) | (uint64_t)(sb + 1u);
+		uint64_t nh = (((h >> 32) + 1ull) << 32) | (uint64_t)(first + 1u);
 		uint64_t seen = cas(hp, h, nh);
 		if (seen == h)
 			break;
@@ -346,7 +361,12 @@ __device__ inline void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb
 	}
 	if ((uint32_t)h == 0)
 		f_or(pmask(H, shard / shards_per_xcd(H)), shard_bit(H, shard));
-	fadd(hp + 1, (uint64_t)1);
+	fadd(hp + 1, (uint64_t)n);
+}
+
+__device__ __forceinline__ void pool_push(const kma_dheap *H, uint32_t shard, uint32_t sb)
+{
+	pool_push_chain(H, shard, sb, sb, 1u);
 }
 
 __device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32_t *out)
@@ -401,9 +421,56 @@ __device__ inline bool pool_pop_bits(const kma_dheap *H, uint32_t y, uint64_t bi
 	return false;
 }
 
-/* Pop for XCD x: its own clean shards, then its own dirty shards, then the
- * other XCDs' clean shards.  The eight shard masks are read in one round
- * trip, so an exhausted pool fails fast. */
+/* Make up to KMA_MIGRATE_MAX of XCD x's dirty superblocks usable by every
+ * XCD (runs on x, one lane): pop them, chaining them through pnext, write
+ * back x's L2 once, push the chain onto one of x's clean shards.  Every
+ * store to a popped superblock completed in x's L2 (or was written back by a
+ * remote freer) before its push, which the pop observed, so the one
+ * write-back covers the whole batch. */
+#define KMA_MIGRATE_MAX 64u
+__device__ inline void migrate_dirty(const kma_dheap *H, uint32_t x, uint32_t ctx)
+{
+	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
+	uint64_t mask = ld(pmask(H, x)) & ((1ull << D) - 1ull);
+	uint32_t first = 0, last = 0, n = 0, sb;
+	for (uint32_t i = 0; i < D && n < KMA_MIGRATE_MAX; i++) {
+		uint32_t k = (ctx + i) % D;
+		if (!((mask >> k) & 1ull))
+			continue;
+		while (n < KMA_MIGRATE_MAX && pool_pop_shard(H, x * S + k, &sb)) {
+			xchg(pnext(H, sb), first);
+			if (n == 0)
+				last = sb;
+			first = sb + 1u;
+			n++;
+		}
+	}
+	if (n == 0)
+		return;
+	release_l2();
+	pool_push_chain(H, clean_shard(H, x, ctx), first - 1u, last, n);
+	fadd(stats(H) + KMA_STAT_MIGRATED, (uint64_t)n);
+}
+
+/* Run a migration if another XCD asked XCD x for one (rq: the request word
+ * as read by the caller); the lane that clears the bit does it. */
+__device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint32_t ctx, uint64_t rq)
+{
+	uint64_t bit = 1ull << x;
+	if ((rq & bit) && (f_and(creq(H), ~bit) & bit))
+		migrate_dirty(H, x, ctx);
+}
+
+/* Pop for XCD x: its own clean shards, its own dirty shards, the other XCDs'
+ * clean shards (clean first: popping the dirty shards that this XCD's
+ * releases are pushing onto measured slower).
+ * The eight shard masks and the request word are read in one round trip, so
+ * an exhausted pool fails fast.  With no clean superblock anywhere, a caller
+ * that newly requests a migration from the XCDs holding dirty ones waits a
+ * bounded time for one (serving requests aimed at its own XCD meanwhile);
+ * a request already pending means those XCDs have not answered, and the
+ * caller fails at once. */
+#define KMA_CREQ_POLLS 16u
 __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t *out)
 {
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
@@ -413,16 +480,35 @@ __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, ui
 #pragma unroll
 	for (uint32_t y = 0; y < KMA_NXCD; y++)
 		m[y] = ld(pmask(H, y));
-	if (pool_pop_bits(H, x, m[x] & clean, D + ctx % (S - D), out))
-		return true;
-	if (pool_pop_bits(H, x, m[x] & dirty, ctx % D, out))
-		return true;
-	for (uint32_t j = 1; j < KMA_NXCD; j++) {
-		uint32_t y = (x + j) & (KMA_NXCD - 1u);
-		if (pool_pop_bits(H, y, m[y] & clean, D + ctx % (S - D), out))
+	uint64_t rq = ld(creq(H));
+	serve_creq(H, x, ctx, rq);
+	for (uint32_t poll = 0;; poll++) {
+		if (pool_pop_bits(H, x, m[x] & clean, D + ctx % (S - D), out))
 			return true;
+		if (pool_pop_bits(H, x, m[x] & dirty, ctx % D, out))
+			return true;
+		for (uint32_t j = 1; j < KMA_NXCD; j++) {
+			uint32_t y = (x + j) & (KMA_NXCD - 1u);
+			if (pool_pop_bits(H, y, m[y] & clean, D + ctx % (S - D), out))
+				return true;
+		}
+		if (poll == KMA_CREQ_POLLS)
+			return false;
+		if (poll == 0) {
+			uint64_t want = 0;
+			for (uint32_t y = 0; y < KMA_NXCD; y++)
+				if (y != x && (m[y] & dirty))
+					want |= 1ull << y;
+			if (want == 0 || (rq & want) == want)
+				return false;
+			f_or(creq(H), want);
+		}
+		__builtin_amdgcn_s_sleep(32);
+#pragma unroll
+		for (uint32_t y = 0; y < KMA_NXCD; y++)
+			m[y] = ld(pmask(H, y));
+		serve_creq(H, x, ctx, ld(creq(H)));
 	}
-	return false;
 }
 
 /* Called by whoever produced state `s` == {count 0, not owned, class set},
@@ -434,13 +520,10 @@ __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, 
 		uint64_t fresh = (uint64_t)((g + 1u) & ST_GEN_MASK) << ST_GEN_SHIFT;
 		uint64_t seen = cas(state(H, sb), s, fresh);
 		if (seen == s) {
+			/* a local release leaves this XCD's L2 as it is; a remote
+			 * freer has written its own L2 back (kma_free) */
 			uint32_t home = st_home(s);
-			if (home == x) {
-				release_l2();
-				pool_push(H, clean_shard(H, x, ctx), sb);
-			} else {
-				pool_push(H, dirty_shard(H, home, ctx), sb);
-			}
+			pool_push(H, dirty_shard(H, home, ctx), sb);
 			return;
 		}
 		s = seen;
@@ -465,7 +548,7 @@ struct resv {
 
 /* Reserve up to n slots of class c from context ctx's current superblock
  * (one lane executes this for its wavefront or work-group group). */
-__device__ inline resv reserve(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c, uint32_t n)
+__device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c, uint32_t n)
 {
 	uint64_t *hp = head(H, ctx, c);
 	uint32_t cap = H->class_cap[c];
@@ -797,11 +880,12 @@ __device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
 		s += ld(pool(H, k) + 1);
 	for (int o = 32; o > 0; o >>= 1)
 		s += __shfl_xor(s, o);
-	return s;
+	/* a pop's decrement can land before its push's increment */
+	return (int64_t)s < 0 ? 0u : s;
 }
 
 /* Allocate `size` bytes; NULL for size 0 or when the heap cannot serve it. */
-__device__ inline void *kma_malloc(kma_dheap *H, size_t size)
+__device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 {
 	using namespace kma_nat;
 	if (size == 0)
@@ -846,7 +930,7 @@ __device__ inline void *kma_malloc(kma_dheap *H, size_t size)
 
 /* Free a block obtained from kma_malloc on the same heap; NULL is a no-op.
  * A second free of the same block is detected and counted, not applied. */
-__device__ inline void kma_free(kma_dheap *H, void *ptr)
+__device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 {
 	using namespace kma_nat;
 	if (!ptr)

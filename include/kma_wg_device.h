@@ -147,10 +147,10 @@ __device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
  * size 0 means "no request" for that thread.  per_lane_claim selects one
  * bitmap atomic per block instead of one per bitmap word. */
 template <int BLOCK>
-__device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t per_lane_claim = 0)
+__device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t per_lane_claim = 0)
 {
 	using namespace kma_nat;
-	const uint32_t tid = threadIdx.x, nth = blockDim.x, nw = (nth + 63u) >> 6, w = tid >> 6;
+	const uint32_t tid = threadIdx.x, nth = blockDim.x, nw = (nth + 63u) >> 6;
 	const uint32_t x = xcc(), ctx = wg_ctx_of(H, x);
 	const uint32_t ncls = H->nclasses;
 	if (tid < 32)
@@ -241,7 +241,7 @@ __device__ inline void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOC
 /* Work-group collective free.  All threads call it; NULL means "no
  * request".  Double frees are detected and counted, not applied. */
 template <int BLOCK>
-__device__ inline void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L)
+__device__ __forceinline__ void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L)
 {
 	using namespace kma_nat;
 	const uint32_t tid = threadIdx.x, nth = blockDim.x;

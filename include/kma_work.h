@@ -62,6 +62,8 @@ typedef struct kma_run_stats {
 	uint64_t pool_empty;    /* refills that found no free superblock */
 	uint64_t waits[4];      /* watchdog expiries: reserve, claim, wavefront claim, pool */
 	uint64_t remote_frees;  /* frees issued from a non-home XCD */
+	uint64_t migrated;      /* superblocks moved from an XCD's dirty shards to
+				   its clean shards inside a kernel */
 	double lat_p50_ns;      /* KMA_VAR_LATENCY: malloc latency percentiles */
 	double lat_p99_ns;
 	double lat_max_ns;

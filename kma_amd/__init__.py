@@ -85,6 +85,7 @@ class RunStats(ctypes.Structure):
         ("pool_empty", ctypes.c_uint64),
         ("waits", ctypes.c_uint64 * 4),
         ("remote_frees", ctypes.c_uint64),
+        ("migrated", ctypes.c_uint64),
         ("lat_p50_ns", ctypes.c_double),
         ("lat_p99_ns", ctypes.c_double),
         ("lat_max_ns", ctypes.c_double),

@@ -415,6 +415,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	for (int k = 0; k < 4; k++)
 		out->waits[k] = st1[3 + k] - st0[3 + k];
 	out->remote_frees = st1[7] - st0[7];
+	out->migrated = st1[8] - st0[8];
 	double tick = kma_i_wall_clock_ns();
 	if (hs[7])
 		for (int k = 0; k < 3; k++)

@@ -282,10 +282,11 @@ extern "C" int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, u
  * and allocates / frees them collectively (kma_wg_device.h).  Work-items
  * past nwi take part with empty requests so control flow stays uniform.
  */
+template <bool GENERIC>
 __global__ void __launch_bounds__(KMA_BATCH_BLOCK) k_batched(kma_dheap *H, kma_wl_args a)
 {
 	__shared__ kma_wg_lds<KMA_BATCH_BLOCK> L;
-	wl_batched(H, L, a);
+	wl_batched<GENERIC>(H, L, a);
 }
 
 extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed, uint64_t seed,
@@ -293,8 +294,12 @@ extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int m
 {
 	uint64_t blocks = (nwi + KMA_BATCH_BLOCK - 1) / KMA_BATCH_BLOCK;
 	kma_wl_args a = {round0, nrounds, mode, fixed, seed, nwi, variant, d_stats, d_lat};
-	hipLaunchKernelGGL(k_batched, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, (hipStream_t)stream,
-			   (kma_dheap *)heap, a);
+	if (variant == 0 && !d_lat)
+		hipLaunchKernelGGL(k_batched<false>, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0,
+				   (hipStream_t)stream, (kma_dheap *)heap, a);
+	else
+		hipLaunchKernelGGL(k_batched<true>, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0,
+				   (hipStream_t)stream, (kma_dheap *)heap, a);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }

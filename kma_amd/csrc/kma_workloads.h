@@ -218,9 +218,15 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 
 #define KMA_BATCH_BLOCK 256
 
+/* GENERIC = false compiles the plain collective path only (variant 0, no
+ * latency probes): the per-lane malloc/free and the probes would otherwise
+ * set the register budget, and with it the occupancy, of the whole kernel. */
+template <bool GENERIC>
 __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, const kma_wl_args &a)
 {
-	const uint32_t round0 = a.round0, nrounds = a.nrounds, fixed = a.fixed, variant = a.variant;
+	const uint32_t round0 = a.round0, nrounds = a.nrounds, fixed = a.fixed;
+	const uint32_t variant = GENERIC ? a.variant : 0u;
+	uint32_t *const lat = GENERIC ? a.lat : nullptr;
 	const int mode = a.mode;
 	const uint64_t seed = a.seed, nwi = a.nwi;
 	unsigned long long *stats = a.stats;
@@ -240,13 +246,13 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 		}
 		uint32_t *p;
 		ph.start();
-		uint64_t t0 = a.lat ? wall_clock64() : 0;
+		uint64_t t0 = lat ? wall_clock64() : 0;
 		if (variant & 1u)
 			p = (uint32_t *)kma_malloc(H, size);
 		else
 			p = (uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, (variant >> 2) & 1u);
-		if (a.lat && active)
-			a.lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
+		if (lat && active)
+			lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
 		ph.mark(0);
 		if (active) {
 			mallocs++;

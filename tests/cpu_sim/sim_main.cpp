@@ -153,7 +153,7 @@ int main(int argc, char **argv)
 				g_sim_lane.wg = wgs[b].get();
 				g_sim_lane.wave = waves[b * (KMA_BATCH_BLOCK / 64) + t / 64].get();
 				kma_wl_args wa = {0, rounds, mode, 64, seed, nwi, 0, stats, nullptr};
-				wl_batched(H, *lds[b], wa);
+				wl_batched<true>(H, *lds[b], wa);
 			});
 	for (auto &t : th)
 		t.join();

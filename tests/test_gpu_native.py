@@ -102,6 +102,25 @@ def test_full_occupancy_native(form):
         assert natimg.NatHeap(h).canonical_problems() == []
 
 
+@pytest.mark.parametrize("form", [kma_amd.FORM_PLAIN, kma_amd.FORM_BATCHED])
+def test_cross_xcd_supply_by_migration(form):
+    """A heap of 16 superblocks (two per XCD, four 4 KiB blocks each) under
+    eight work-groups of 4 KiB requests: superblocks emptied on one XCD stay
+    in its dirty shards, so the others get them only through in-kernel
+    migrations (dirty -> clean after one L2 write-back).  Patterns read back,
+    no watchdog fires, and the heap is canonical after trim."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16, nctx=8, npool=16) as h:
+        _say("native cross-XCD supply form=%d" % form)
+        st = h.run(kma_amd.WL_FIXED, 8 * 256, 30, form=form, block=256, fixed=4096)
+        assert st["ops"] >= 30 * 8 * 256 and st["bad"] == 0, st
+        assert st["double_frees"] == 0 and st["bad_frees"] == 0, st
+        assert st["waits"] == [0, 0, 0, 0], st
+        assert 0 < st["nulls"] < st["mallocs"], st
+        assert st["migrated"] > 0, st
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
 def test_batched_plain_small_and_partial_workgroup():
     """Batched form with a work-item count that leaves the last work-group
     partly empty, and the fixed-size workload."""
