@@ -32,7 +32,7 @@ typedef struct kma_config {
 	uint64_t sblocks;     /* number of superblocks */
 	/* NATIVE only (0 selects the default) */
 	uint32_t nctx;        /* allocation contexts (power of two) */
-	uint32_t npool;       /* superblock pool shards (power of two) */
+	uint32_t npool;       /* superblock pool shards (power of two, 16..512; 0: min(2 * nctx, 512)) */
 	uint32_t nclasses;    /* entries used in class_size[] */
 	uint32_t class_size[KMA_MAX_CLASSES]; /* ascending block sizes, multiples of 8 */
 	uint32_t page_log2;   /* large-request page size log2 (default = sb_log2) */

@@ -106,11 +106,11 @@ typedef struct kma_dheap {
 #define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
 #define KMA_STAT_MIGRATED 8     /* superblocks moved from dirty to clean shards in a kernel */
 #define KMA_NSTATS 9
-/* The stats block is 256 B of counters, one 128-B line per XCD holding that
- * XCD's shard mask (bit k set when shard x*S + k may hold a superblock, see
- * pool_push / pool_pop_shard) and one 128-B line holding the clean-request
- * word (bit x: another XCD asks XCD x to migrate dirty superblocks). */
-#define KMA_STATS_BYTES (256u + 128u * 9u)
+/* The stats block is 256 B of counters and one 128-B line per XCD x holding
+ * x's shard mask (bit k set when shard x*S + k may hold a superblock, see
+ * pool_push / pool_pop_shard) and, in the next word, x's clean request
+ * (non-zero: another XCD asks x to migrate dirty superblocks). */
+#define KMA_STATS_BYTES (256u + 128u * 8u)
 /* Page runs of k pages use an aligned window of 2^l >= k pages inside one
  * 64-page bitmap word; levels l = 0..6. */
 #define KMA_PAGE_LEVELS 7u
@@ -249,7 +249,7 @@ __device__ __forceinline__ uint8_t *plen(const kma_dheap *H) { return (uint8_t *
 __device__ __forceinline__ char *sbdata(const kma_dheap *H, uint64_t sb) { return base(H) + H->off_data + (sb << H->sb_log2); }
 __device__ __forceinline__ uint64_t *stats(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats); }
 __device__ __forceinline__ uint64_t *pmask(const kma_dheap *H, uint32_t x) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * x); }
-__device__ __forceinline__ uint64_t *creq(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * KMA_NXCD); }
+__device__ __forceinline__ uint64_t *creq(const kma_dheap *H, uint32_t x) { return pmask(H, x) + 1; }
 __device__ __forceinline__ uint64_t psum_words(const kma_dheap *H) { return (((H->npages + 63) >> 6) + 63) >> 6; }
 __device__ __forceinline__ uint64_t *psum(const kma_dheap *H, uint32_t l) { return (uint64_t *)(base(H) + H->off_psum) + l * psum_words(H); }
 
@@ -297,10 +297,9 @@ __device__ __forceinline__ uint32_t wg_ctx_of(const kma_dheap *H, uint32_t x)
  * must not be written back over another XCD's later use.  Another XCD gets
  * such superblocks only through a migration run on x (migrate_dirty): pop a
  * batch, write back x's L2 once, push the batch onto a clean shard.  An XCD
- * that finds no clean superblock anywhere sets the bits of the XCDs holding
- * dirty ones in the clean-request word; those XCDs migrate when they next
- * pop a superblock, and kma_trim (after a kernel boundary) empties every
- * dirty shard.
+ * that finds no clean superblock anywhere sets the clean request of each XCD
+ * holding dirty ones; those XCDs migrate when they next pop a superblock,
+ * and kma_trim (after a kernel boundary) empties every dirty shard.
  * pmask(x) bit k is a hint that shard x*S + k is non-empty, updated only
  * when a shard goes empty <-> non-empty:
  *   - a push that found the shard empty sets the bit after its CAS;
@@ -357,7 +356,7 @@ __device__ inline void pool_push_chain(const kma_dheap *H, uint32_t shard, uint3
 		h = seen;
 		for (uint32_t i = 0; i < bo; i++)
 			__builtin_amdgcn_s_sleep(1);
-		bo = bo < 16u ? 2u * bo : 16u;
+		bo = bo < 2u ? 2u * bo : 2u;
 	}
 	if ((uint32_t)h == 0)
 		f_or(pmask(H, shard / shards_per_xcd(H)), shard_bit(H, shard));
@@ -398,7 +397,7 @@ __device__ inline bool pool_pop_shard(const kma_dheap *H, uint32_t shard, uint32
 		h = seen;
 		for (uint32_t i = 0; i < bo; i++)
 			__builtin_amdgcn_s_sleep(1);
-		bo = bo < 16u ? 2u * bo : 16u;
+		bo = bo < 2u ? 2u * bo : 2u;
 	}
 }
 
@@ -452,23 +451,23 @@ __device__ inline void migrate_dirty(const kma_dheap *H, uint32_t x, uint32_t ct
 	fadd(stats(H) + KMA_STAT_MIGRATED, (uint64_t)n);
 }
 
-/* Run a migration if another XCD asked XCD x for one (rq: the request word
- * as read by the caller); the lane that clears the bit does it. */
+/* Run a migration if another XCD asked XCD x for one (rq: x's request word
+ * as read by the caller); the lane that clears the request does it. */
 __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint32_t ctx, uint64_t rq)
 {
-	uint64_t bit = 1ull << x;
-	if ((rq & bit) && (f_and(creq(H), ~bit) & bit))
+	if (rq && xchg(creq(H, x), (uint64_t)0))
 		migrate_dirty(H, x, ctx);
 }
 
 /* Pop for XCD x: its own clean shards, its own dirty shards, the other XCDs'
  * clean shards (clean first: popping the dirty shards that this XCD's
- * releases are pushing onto measured slower).
- * The eight shard masks and the request word are read in one round trip, so
- * an exhausted pool fails fast.  With no clean superblock anywhere, a caller
- * that newly requests a migration from the XCDs holding dirty ones waits a
+ * releases are pushing onto measured slower).  A pop reads only its own
+ * XCD's mask/request line until its own shards are exhausted (every pop of
+ * the device reading all eight masks made them hot), then the other masks
+ * in one round trip.  With no clean superblock anywhere, a caller that
+ * newly requests a migration from the XCDs holding dirty ones waits a
  * bounded time for one (serving requests aimed at its own XCD meanwhile);
- * a request already pending means those XCDs have not answered, and the
+ * requests already pending mean those XCDs have not answered, and the
  * caller fails at once. */
 #define KMA_CREQ_POLLS 16u
 __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t *out)
@@ -476,17 +475,24 @@ __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, ui
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
 	uint64_t all = S == 64 ? ~0ull : ((1ull << S) - 1ull);
 	uint64_t dirty = (1ull << D) - 1ull, clean = all & ~dirty;
+	uint64_t mx = ld(pmask(H, x));
+	serve_creq(H, x, ctx, ld(creq(H, x)));
+	if (pool_pop_bits(H, x, mx & clean, D + ctx % (S - D), out))
+		return true;
+	if (pool_pop_bits(H, x, mx & dirty, ctx % D, out))
+		return true;
 	uint64_t m[KMA_NXCD];
-#pragma unroll
-	for (uint32_t y = 0; y < KMA_NXCD; y++)
-		m[y] = ld(pmask(H, y));
-	uint64_t rq = ld(creq(H));
-	serve_creq(H, x, ctx, rq);
 	for (uint32_t poll = 0;; poll++) {
-		if (pool_pop_bits(H, x, m[x] & clean, D + ctx % (S - D), out))
-			return true;
-		if (pool_pop_bits(H, x, m[x] & dirty, ctx % D, out))
-			return true;
+#pragma unroll
+		for (uint32_t y = 0; y < KMA_NXCD; y++)
+			m[y] = ld(pmask(H, y));
+		if (poll > 0) {
+			serve_creq(H, x, ctx, ld(creq(H, x)));
+			if (pool_pop_bits(H, x, m[x] & clean, D + ctx % (S - D), out))
+				return true;
+			if (pool_pop_bits(H, x, m[x] & dirty, ctx % D, out))
+				return true;
+		}
 		for (uint32_t j = 1; j < KMA_NXCD; j++) {
 			uint32_t y = (x + j) & (KMA_NXCD - 1u);
 			if (pool_pop_bits(H, y, m[y] & clean, D + ctx % (S - D), out))
@@ -495,19 +501,14 @@ __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, ui
 		if (poll == KMA_CREQ_POLLS)
 			return false;
 		if (poll == 0) {
-			uint64_t want = 0;
+			uint32_t asked = 0;
 			for (uint32_t y = 0; y < KMA_NXCD; y++)
-				if (y != x && (m[y] & dirty))
-					want |= 1ull << y;
-			if (want == 0 || (rq & want) == want)
+				if (y != x && (m[y] & dirty) && !xchg(creq(H, y), (uint64_t)1))
+					asked++;
+			if (asked == 0)
 				return false;
-			f_or(creq(H), want);
 		}
 		__builtin_amdgcn_s_sleep(32);
-#pragma unroll
-		for (uint32_t y = 0; y < KMA_NXCD; y++)
-			m[y] = ld(pmask(H, y));
-		serve_creq(H, x, ctx, ld(creq(H)));
 	}
 }
 
@@ -587,8 +588,8 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 		}
 		if (h == HEAD_BUSY) {
 			for (uint32_t i = 0; i < bo; i++)
-				__builtin_amdgcn_s_sleep(4);
-			bo = bo < 64u ? 2u * bo : 64u;
+				__builtin_amdgcn_s_sleep(2);
+			bo = bo < 4u ? 2u * bo : 4u;
 			h = fadd(hp, (uint64_t)0);
 			continue;
 		}
@@ -621,28 +622,37 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 }
 
 /* Claim one free bit at or after `start` (wrapping) in superblock sb.  The
- * first pass reads the bitmap with plain atomic loads; later passes read it
- * with RMWs, so a stale cached word cannot hide free bits for long. */
+ * reserved slot `start` itself is tried first with no read (blocks are
+ * claimed bottom-up and mostly freed in bulk, so it is usually free); then
+ * the scan continues from the word the RMW returned.  Words are read with
+ * plain atomic loads; after a wrap they are re-read with RMWs, so a stale
+ * cached word cannot hide free bits for long. */
 __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t start)
 {
 	uint32_t cap = H->class_cap[c];
 	uint32_t *bm = bitmap(H, sb);
-	uint32_t b = start;
 	uint32_t lastw = (cap - 1u) >> 5;
 	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
+	uint32_t b = start < cap ? start : 0u;
+	uint32_t wi = b >> 5;
+	uint32_t m = 1u << (b & 31u);
+	uint32_t w = f_or(bm + wi, m);
+	if (!(w & m))
+		return sbdata(H, sb) + (uint64_t)b * H->class_size[c];
+	w |= m;
 	bool fresh = false;
 	waiter wt;
 	while (!wt.expired()) {
-		uint32_t wi = b >> 5;
-		uint32_t w = fresh ? f_or(bm + wi, 0u) : ld(bm + wi);
 		uint32_t avail = ~w & (~0u << (b & 31u));
 		if (wi == lastw)
 			avail &= tail;
 		if (avail) {
 			uint32_t bit = __ffs(avail) - 1u;
-			uint32_t m = 1u << bit;
-			if (!(f_or(bm + wi, m) & m))
+			m = 1u << bit;
+			uint32_t old = f_or(bm + wi, m);
+			if (!(old & m))
 				return sbdata(H, sb) + (uint64_t)(wi * 32u + bit) * H->class_size[c];
+			w = old | m;
 			b = wi * 32u + bit;
 			continue;
 		}
@@ -651,6 +661,8 @@ __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint3
 			b = 0;
 			fresh = true;
 		}
+		wi = b >> 5;
+		w = fresh ? f_or(bm + wi, 0u) : ld(bm + wi);
 	}
 	bump_stat(H, KMA_STAT_WAIT_CLAIM);
 	return nullptr;

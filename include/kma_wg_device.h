@@ -72,11 +72,24 @@ __device__ __forceinline__ uint32_t group_bcast(uint32_t v)
 	return __shfl(v, 0, (int)KMA_CLAIM_G);
 }
 
+/* Bits [lo, hi) that fall in bitmap word wi. */
+__device__ __forceinline__ uint32_t range_bits(uint32_t wi, uint32_t lo, uint32_t hi)
+{
+	uint32_t a = max(lo, wi * 32u), b = min(hi, wi * 32u + 32u);
+	if (a >= b)
+		return 0u;
+	uint32_t n = b - a;
+	return (n == 32u ? ~0u : ((1u << n) - 1u)) << (a - wi * 32u);
+}
+
 /* Each group with want set claims `got` free bits of its superblock sb
- * (class c), scanning from the word that holds bit `base`, 16 words per
- * pass, and writes block offsets (from the heap base) into
- * res[order[0..done-1]]; returns the group's `done`.  After a full pass
- * over the bitmap without progress the words are re-read with RMWs. */
+ * (class c) and writes block offsets (from the heap base) into
+ * res[order[0..done-1]]; returns the group's `done`.  The first pass claims
+ * the reserved slots [base, base + got) themselves with no read (blocks are
+ * claimed bottom-up and mostly freed in bulk, so they are usually free; the
+ * RMWs say which were).  Later passes read the words from the one holding
+ * bit `base`, 16 words per pass, and take the lowest free bits; after a full
+ * pass over the bitmap without progress the words are re-read with RMWs. */
 __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t sb, uint32_t c, uint32_t base,
 				       uint32_t got, const uint16_t *order, uint64_t *res)
 {
@@ -89,7 +102,7 @@ __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t s
 	uint32_t bsize = want ? H->class_size[c] : 0u;
 	uint32_t w0 = want ? (base >> 5) % W : 0u;
 	uint32_t done = 0, chunk = 0, before = 0;
-	bool fresh = false, need = want && got > 0;
+	bool fresh = false, opt = true, need = want && got > 0;
 	waiter wt;
 	while (__ballot(need)) {
 		uint32_t k = chunk + gl;
@@ -97,7 +110,10 @@ __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t s
 		uint32_t wi = has ? (w0 + k) % W : 0u;
 		uint32_t freeb = 0;
 		if (has) {
-			freeb = ~(fresh ? f_or(bm + wi, 0u) : ld(bm + wi));
+			if (opt)
+				freeb = range_bits(wi, base, min(base + got, cap));
+			else
+				freeb = ~(fresh ? f_or(bm + wi, 0u) : ld(bm + wi));
 			if (wi == W - 1u)
 				freeb &= tail;
 		}
@@ -119,7 +135,10 @@ __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t s
 		const bool exp = group_bcast(need && wt.expired() ? 1u : 0u) != 0;
 		if (need) {
 			done += wtot;
-			chunk += KMA_CLAIM_G;
+			if (opt)
+				opt = false; /* then scan from word w0 */
+			else
+				chunk += KMA_CLAIM_G;
 			if (chunk >= W) {
 				/* a full pass: re-read with RMWs if it made no progress */
 				chunk = 0;
@@ -134,6 +153,55 @@ __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t s
 		}
 	}
 	return done;
+}
+
+/* Whole-superblock grab (every lane of the wavefront calls; lanes with
+ * want set each pop one superblock for class c and claim all `cap` slots).
+ * The successful lanes of each group take consecutive rank ranges of `cap`
+ * from rank 0; returns the group's number of ranks covered.  A pooled
+ * superblock has count 0 and a clear bitmap, so every RMW should win; a
+ * slot that does not (a corrupt heap) is left out and its count given back,
+ * and its rank gets NULL. */
+#define KMA_WHOLE_WORDS 8u
+__device__ __forceinline__ uint32_t whole_superblocks(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c,
+					     uint32_t cap, bool want, const uint16_t *order, uint64_t *res)
+{
+	uint32_t sb = 0;
+	bool have = want && pool_pop(H, x, ctx + grp_lane(), &sb);
+	uint32_t old[KMA_WHOLE_WORDS];
+	const uint32_t W = (cap + 31u) >> 5;
+	const uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
+	if (have) {
+		fadd(state(H, sb), ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ((uint64_t)x << ST_HOME_SHIFT) | cap);
+		uint32_t *bm = bitmap(H, sb);
+#pragma unroll
+		for (uint32_t i = 0; i < KMA_WHOLE_WORDS; i++)
+			if (i < W)
+				old[i] = f_or(bm + i, i == W - 1u ? tail : ~0u);
+	}
+	uint32_t tot;
+	const uint32_t idx = group_excl_scan(have ? 1u : 0u, &tot);
+	if (have) {
+		const uint64_t d0 = H->off_data + ((uint64_t)sb << H->sb_log2);
+		const uint32_t bsize = H->class_size[c];
+		uint32_t lost = 0;
+#pragma unroll
+		for (uint32_t i = 0; i < KMA_WHOLE_WORDS; i++) {
+			if (i >= W)
+				continue;
+			uint32_t full = i == W - 1u ? tail : ~0u;
+			for (uint32_t b = 0; b < 32u && i * 32u + b < cap; b++) {
+				bool won = ((full & ~old[i]) >> b) & 1u;
+				lost += won ? 0u : 1u;
+				res[order[idx * cap + i * 32u + b]] = won ? d0 + (uint64_t)(i * 32u + b) * bsize : 0u;
+			}
+		}
+		if (lost) {
+			bump_stat(H, KMA_STAT_WAIT_WGCLAIM);
+			undo(H, sb, lost, x, ctx);
+		}
+	}
+	return tot * cap;
 }
 
 __device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
@@ -185,7 +253,14 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 		const uint32_t n = k < ncls ? L.cnt[k] : 0u;
 		const uint16_t *ord = n ? &L.order[L.cbase[k]] : L.order;
 		uint32_t r0 = 0;
-		bool pend = n > 0;
+		/* a class with at least a superblock's worth of requests takes whole
+		 * superblocks straight from the pool, one per group lane, all in
+		 * flight at once, and claims every slot with no reads */
+		const uint32_t cap = n ? H->class_cap[k] : 0u;
+		const uint32_t nwhole = (n && cap > 1u && cap <= 32u * KMA_WHOLE_WORDS) ? min(n / cap, G) : 0u;
+		if (__ballot(nwhole != 0u))
+			r0 = whole_superblocks(H, x, ctx, k, cap, gl < nwhole, ord, L.res);
+		bool pend = r0 < n;
 		while (__ballot(pend)) {
 			resv r = {0, 0, 0, 0};
 			if (pend && gl == 0)

@@ -124,14 +124,15 @@ __global__ void k_trim(kma_dheap *H, uint32_t nctx, uint32_t nclasses)
 }
 
 /* After a kernel boundary every L2 has been written back, so superblocks in
- * dirty shards may go to any XCD: move them to their XCD's clean shards
- * (one thread; no allocation may be in flight). */
+ * dirty shards may go to any XCD: move them to their XCD's clean shards and
+ * drop pending clean requests (one thread; no allocation may be in flight). */
 __global__ void k_clean_dirty(kma_dheap *H)
 {
 	using namespace kma_nat;
 	if (blockIdx.x != 0 || threadIdx.x != 0)
 		return;
 	for (uint32_t x = 0; x < KMA_NXCD; x++) {
+		st(creq(H, x), (uint64_t)0);
 		uint32_t sb, n = 0;
 		for (uint32_t k = 0; k < dirty_per_xcd(H); k++)
 			while (pool_pop_shard(H, dirty_shard(H, x, k), &sb))

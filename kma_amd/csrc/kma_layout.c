@@ -65,7 +65,7 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 		return -1;
 	}
 	if (cfg->npool == 0)
-		cfg->npool = cfg->nctx * 2 < 256 ? cfg->nctx * 2 : 256;
+		cfg->npool = cfg->nctx * 2 < 512 ? cfg->nctx * 2 : 512;
 	if (!is_pow2(cfg->npool) || cfg->npool < 16 || cfg->npool > 512) {
 		snprintf(err, errlen, "native layout: npool %u must be a power of two in 16..512", cfg->npool);
 		return -1;

@@ -47,7 +47,7 @@ typedef struct nat_hdr {
 #define NAT_MAGIC 0x4e414d4b31414d4bull
 #define NAT_HEADS_PER_CTX 32u
 #define NAT_POOL_STRIDE_BYTES 128u
-#define NAT_STATS_BYTES (256u + 128u * 9u) /* counters, one line per XCD, the request line */
+#define NAT_STATS_BYTES (256u + 128u * 8u) /* counters + one line per XCD (shard mask, clean request) */
 #define NAT_PAGE_LEVELS 7u
 
 /* Validate `in`, fill defaults into *out and the header (metadata offsets;

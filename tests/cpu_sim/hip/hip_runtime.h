@@ -132,6 +132,9 @@ static inline int __clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
 static inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 static inline uint64_t min(uint64_t a, uint64_t b) { return a < b ? a : b; }
 static inline int min(int a, int b) { return a < b ? a : b; }
+static inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
+static inline uint64_t max(uint64_t a, uint64_t b) { return a > b ? a : b; }
+static inline int max(int a, int b) { return a > b ? a : b; }
 
 #define __HIP_MEMORY_SCOPE_WORKGROUP 2
 #define __HIP_MEMORY_SCOPE_AGENT 3

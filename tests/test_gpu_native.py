@@ -108,7 +108,10 @@ def test_cross_xcd_supply_by_migration(form):
     eight work-groups of 4 KiB requests: superblocks emptied on one XCD stay
     in its dirty shards, so the others get them only through in-kernel
     migrations (dirty -> clean after one L2 write-back).  Patterns read back,
-    no watchdog fires, and the heap is canonical after trim."""
+    no watchdog fires, and the heap is canonical after trim.  The per-lane
+    form requests migrations (its superblocks idle in dirty shards while the
+    owner runs); the collective form takes whole superblocks and mostly
+    re-pops its own dirty ones, so migrations there are timing-dependent."""
     with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16, nctx=8, npool=16) as h:
         _say("native cross-XCD supply form=%d" % form)
         st = h.run(kma_amd.WL_FIXED, 8 * 256, 30, form=form, block=256, fixed=4096)
@@ -116,7 +119,8 @@ def test_cross_xcd_supply_by_migration(form):
         assert st["double_frees"] == 0 and st["bad_frees"] == 0, st
         assert st["waits"] == [0, 0, 0, 0], st
         assert 0 < st["nulls"] < st["mallocs"], st
-        assert st["migrated"] > 0, st
+        if form == kma_amd.FORM_PLAIN:
+            assert st["migrated"] > 0, st
         h.trim()
         assert natimg.NatHeap(h).canonical_problems() == []
 
@@ -158,11 +162,17 @@ def test_batched_mixed_with_per_lane_api(variant):
 
 
 def _expected_claim(bm, cap, base, got, d0, size):
-    """Model of the wavefront claim with no concurrent claimer: words from
-    (base/32) mod W onward, lowest free bits first, in rank order."""
+    """Model of the group claim with no concurrent claimer: first the free
+    bits among the reserved slots [base, base + got) (clipped to the
+    capacity), then words from (base/32) mod W onward, lowest free bits
+    first; ranks in that order."""
     W = (cap + 31) // 32
     w0 = (base >> 5) % W
     out, bm = [], list(bm)
+    for b in range(base, min(base + got, cap)):
+        if not (bm[b >> 5] >> (b & 31)) & 1:
+            bm[b >> 5] |= 1 << (b & 31)
+            out.append(d0 + b * size)
     for k in range(W):
         wi = (w0 + k) % W
         valid = cap - wi * 32
