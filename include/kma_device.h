@@ -92,7 +92,10 @@ typedef struct kma_dheap {
 	uint32_t class_cap[32];
 	uint64_t class_magic[32]; /* ceil(2^40 / size): exact block index */
 	uint8_t class_for_log2[64];
-	uint64_t off_psum; /* page-window summaries, KMA_PAGE_LEVELS levels */
+	uint64_t off_psum;     /* page-window summaries, KMA_PAGE_LEVELS levels */
+	uint64_t off_spare;    /* context spare lines, 128 B per context */
+	uint32_t spare_slots;  /* slots used per spare line (0..16) */
+	uint32_t pad1;
 } kma_dheap;
 
 /* Per-heap counters (u64 at off_stats). */
@@ -420,6 +423,71 @@ __device__ inline bool pool_pop_bits(const kma_dheap *H, uint32_t y, uint64_t bi
 	return false;
 }
 
+/* ---- context spares ----
+ * Each context has a 128-B line of spare slots (u32 sb + 1, 0: empty; the
+ * first spare_slots of them are used): empty superblocks the context keeps
+ * for its next refill, so that most superblock turnover touches only the
+ * context's own line instead of the shared pool shards.  A spare is in the
+ * same state as a pooled superblock (count 0, no class, generation bumped)
+ * and, like a dirty shard's, is reused only on its context's XCD: a pool
+ * pop that finds its own XCD's shards empty also looks into a few other
+ * contexts' lines of that XCD, a migration drains the XCD's lines, and
+ * kma_trim empties them all.  Capacity is at most a quarter of the heap. */
+__device__ __forceinline__ uint32_t *spare(const kma_dheap *H, uint32_t ctx)
+{
+	return (uint32_t *)(base(H) + H->off_spare) + (uint64_t)ctx * 32u;
+}
+
+/* Keep empty superblock sb as a spare of context ctx: a CAS into slot
+ * sb mod n, then into the slot half a line away; false if both are taken. */
+__device__ __forceinline__ bool spare_put(const kma_dheap *H, uint32_t ctx, uint32_t sb)
+{
+	uint32_t n = H->spare_slots;
+	if (n == 0u)
+		return false;
+	uint32_t *sp = spare(H, ctx);
+	uint32_t k = sb % n;
+	if (cas(sp + k, 0u, sb + 1u) == 0u)
+		return true;
+	return n > 1u && cas(sp + (k + n / 2u) % n, 0u, sb + 1u) == 0u;
+}
+
+/* Occupied slots of ctx's line (bit k: slot k), read in one round trip. */
+__device__ __forceinline__ uint32_t spare_occupied(const kma_dheap *H, uint32_t ctx)
+{
+	uint32_t n = H->spare_slots, occ = 0;
+	const uint64_t *sp = (const uint64_t *)spare(H, ctx);
+#pragma unroll
+	for (uint32_t i = 0; i < 8u; i++) {
+		if (2u * i < n) {
+			uint64_t v = ld(sp + i);
+			occ |= ((uint32_t)v != 0u ? 1u : 0u) << (2u * i);
+			occ |= ((uint32_t)(v >> 32) != 0u ? 1u : 0u) << (2u * i + 1u);
+		}
+	}
+	return occ & ((1u << n) - 1u);
+}
+
+/* Swap out an occupied slot of ctx's line (occ: as read), preferring slots
+ * from `start` on; false if every candidate was emptied meanwhile. */
+__device__ inline bool spare_take(const kma_dheap *H, uint32_t ctx, uint32_t occ, uint32_t start, uint32_t *out)
+{
+	uint32_t n = H->spare_slots;
+	uint32_t *sp = spare(H, ctx);
+	uint32_t r = n ? start % n : 0u;
+	while (occ) {
+		uint32_t rot = r ? ((occ >> r) | (occ << (n - r))) & ((1u << n) - 1u) : occ;
+		uint32_t k = (__ffs(rot) - 1u + r) % n;
+		occ &= ~(1u << k);
+		uint32_t v = xchg(sp + k, 0u);
+		if (v) {
+			*out = v - 1u;
+			return true;
+		}
+	}
+	return false;
+}
+
 /* Make up to KMA_MIGRATE_MAX of XCD x's dirty superblocks usable by every
  * XCD (runs on x, one lane): pop them, chaining them through pnext, write
  * back x's L2 once, push the chain onto one of x's clean shards.  Every
@@ -444,6 +512,20 @@ __device__ inline void migrate_dirty(const kma_dheap *H, uint32_t x, uint32_t ct
 			n++;
 		}
 	}
+	/* then the spares of x's contexts */
+	uint32_t per = H->nctx / KMA_NXCD;
+	for (uint32_t i = 0; i < per && n < KMA_MIGRATE_MAX && H->spare_slots; i++) {
+		uint32_t c2 = x * per + (ctx + i) % per;
+		uint32_t occ = spare_occupied(H, c2);
+		while (n < KMA_MIGRATE_MAX && spare_take(H, c2, occ, 0u, &sb)) {
+			occ = spare_occupied(H, c2);
+			xchg(pnext(H, sb), first);
+			if (n == 0)
+				last = sb;
+			first = sb + 1u;
+			n++;
+		}
+	}
 	if (n == 0)
 		return;
 	release_l2();
@@ -459,9 +541,11 @@ __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint3
 		migrate_dirty(H, x, ctx);
 }
 
-/* Pop for XCD x: its own clean shards, its own dirty shards, the other XCDs'
- * clean shards (clean first: popping the dirty shards that this XCD's
- * releases are pushing onto measured slower).  A pop reads only its own
+/* Pop for XCD x and context ctx: ctx's spares, x's clean shards, x's dirty
+ * shards, a few other contexts' spares on x, the other XCDs' clean shards
+ * (clean before dirty: popping the dirty shards that this XCD's releases
+ * are pushing onto measured slower).  `salt` spreads concurrent callers of
+ * one context over slots and shards.  A pop reads only its own
  * XCD's mask/request line until its own shards are exhausted (every pop of
  * the device reading all eight masks made them hot), then the other masks
  * in one round trip.  With no clean superblock anywhere, a caller that
@@ -470,17 +554,36 @@ __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint3
  * requests already pending mean those XCDs have not answered, and the
  * caller fails at once. */
 #define KMA_CREQ_POLLS 16u
-__device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t *out)
+#define KMA_STEAL_CTX 8u
+__device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t salt, uint32_t *out)
 {
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
 	uint64_t all = S == 64 ? ~0ull : ((1ull << S) - 1ull);
 	uint64_t dirty = (1ull << D) - 1ull, clean = all & ~dirty;
+	uint32_t sh = ctx + salt;
 	uint64_t mx = ld(pmask(H, x));
-	serve_creq(H, x, ctx, ld(creq(H, x)));
-	if (pool_pop_bits(H, x, mx & clean, D + ctx % (S - D), out))
+	uint64_t rq = ld(creq(H, x));
+	uint32_t occ = H->spare_slots ? spare_occupied(H, ctx) : 0u;
+	serve_creq(H, x, ctx, rq);
+	if (occ && spare_take(H, ctx, occ, salt, out))
 		return true;
-	if (pool_pop_bits(H, x, mx & dirty, ctx % D, out))
+	if (pool_pop_bits(H, x, mx & clean, D + sh % (S - D), out))
 		return true;
+	if (pool_pop_bits(H, x, mx & dirty, sh % D, out))
+		return true;
+	/* other contexts' spares on this XCD, a few per call */
+	uint32_t per = H->nctx / KMA_NXCD;
+	if (H->spare_slots && per > 1u) {
+		uint32_t o[KMA_STEAL_CTX];
+		uint32_t c0 = (uint32_t)wall_clock64() + salt;
+#pragma unroll
+		for (uint32_t i = 0; i < KMA_STEAL_CTX; i++)
+			o[i] = i < per - 1u ? spare_occupied(H, x * per + (ctx - x * per + 1u + (c0 + i) % (per - 1u)) % per) : 0u;
+#pragma unroll
+		for (uint32_t i = 0; i < KMA_STEAL_CTX; i++)
+			if (o[i] && spare_take(H, x * per + (ctx - x * per + 1u + (c0 + i) % (per - 1u)) % per, o[i], salt, out))
+				return true;
+	}
 	uint64_t m[KMA_NXCD];
 	for (uint32_t poll = 0;; poll++) {
 #pragma unroll
@@ -488,14 +591,14 @@ __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, ui
 			m[y] = ld(pmask(H, y));
 		if (poll > 0) {
 			serve_creq(H, x, ctx, ld(creq(H, x)));
-			if (pool_pop_bits(H, x, m[x] & clean, D + ctx % (S - D), out))
+			if (pool_pop_bits(H, x, m[x] & clean, D + sh % (S - D), out))
 				return true;
-			if (pool_pop_bits(H, x, m[x] & dirty, ctx % D, out))
+			if (pool_pop_bits(H, x, m[x] & dirty, sh % D, out))
 				return true;
 		}
 		for (uint32_t j = 1; j < KMA_NXCD; j++) {
 			uint32_t y = (x + j) & (KMA_NXCD - 1u);
-			if (pool_pop_bits(H, y, m[y] & clean, D + ctx % (S - D), out))
+			if (pool_pop_bits(H, y, m[y] & clean, D + sh % (S - D), out))
 				return true;
 		}
 		if (poll == KMA_CREQ_POLLS)
@@ -524,7 +627,8 @@ __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, 
 			/* a local release leaves this XCD's L2 as it is; a remote
 			 * freer has written its own L2 back (kma_free) */
 			uint32_t home = st_home(s);
-			pool_push(H, dirty_shard(H, home, ctx), sb);
+			if (home != x || !spare_put(H, ctx, sb))
+				pool_push(H, dirty_shard(H, home, ctx), sb);
 			return;
 		}
 		s = seen;
@@ -568,7 +672,7 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 				continue;
 			}
 			uint32_t sb;
-			if (!pool_pop(H, x, ctx, &sb)) {
+			if (!pool_pop(H, x, ctx, 0u, &sb)) {
 				st(hp, HEAD_EMPTY);
 				bump_stat(H, KMA_STAT_POOL_EMPTY);
 				return resv{0, 0, 0, 0};
@@ -826,6 +930,28 @@ __device__ inline void page_free(const kma_dheap *H, char *p)
 	psum_gained(H, wi, old, old & ~m);
 }
 
+/* After a kernel boundary every L2 has been written back, so superblocks in
+ * dirty shards and context spare lines may go to any XCD: move them to
+ * their XCD's clean shards and drop pending clean requests (one thread; no
+ * allocation may be in flight). */
+__device__ inline void clean_all(const kma_dheap *H)
+{
+	uint32_t per = H->nctx / KMA_NXCD;
+	for (uint32_t x = 0; x < KMA_NXCD; x++) {
+		st(creq(H, x), (uint64_t)0);
+		uint32_t sb, n = 0;
+		for (uint32_t k = 0; k < dirty_per_xcd(H); k++)
+			while (pool_pop_shard(H, dirty_shard(H, x, k), &sb))
+				pool_push(H, clean_shard(H, x, n++), sb);
+		for (uint32_t c = x * per; c < (x + 1) * per; c++)
+			for (uint32_t k = 0; k < H->spare_slots; k++) {
+				uint32_t v = xchg(spare(H, c) + k, 0u);
+				if (v)
+					pool_push(H, clean_shard(H, x, n++), v - 1u);
+			}
+	}
+}
+
 /* ---- initialisation (every metadata word starts at zero) ----
  * Item i of max(nsb, npool, KMA_PAGE_LEVELS * psum_words) items:
  *   - superblocks start in the clean shards: clean shard j (of the
@@ -881,15 +1007,32 @@ __host__ __device__ inline uint64_t init_items(uint64_t nsb, uint32_t npool, uin
 
 } /* namespace kma_nat */
 
-/* Superblocks currently in the pool (free for any class): the sum of the
- * shard counts, read by a full wavefront (all 64 lanes call it; every lane
- * gets the sum).  A snapshot: pushes and pops in flight may be missed. */
+/* Superblocks currently free for any class: the sum of the shard counts
+ * plus the occupied context spare slots, read by a full wavefront (all 64
+ * lanes call it; every lane gets the sum).  A snapshot: pushes and pops in
+ * flight may be missed. */
 __device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
 {
 	using namespace kma_nat;
 	uint64_t s = 0;
 	for (uint32_t k = lane(); k < H->npool; k += 64)
 		s += ld(pool(H, k) + 1);
+	const uint32_t n = H->spare_slots, wpc = (n + 1u) / 2u; /* words per line */
+	const uint64_t nw = (uint64_t)H->nctx * wpc;
+	for (uint64_t i = lane(); i < nw; i += 4u * 64u) {
+		uint64_t v[4];
+#pragma unroll
+		for (uint32_t u = 0; u < 4u; u++) {
+			uint64_t j = i + u * 64u;
+			v[u] = j < nw ? ld((const uint64_t *)spare(H, (uint32_t)(j / wpc)) + j % wpc) : 0u;
+		}
+#pragma unroll
+		for (uint32_t u = 0; u < 4u; u++) {
+			uint64_t j = i + u * 64u;
+			s += ((uint32_t)v[u] != 0u ? 1u : 0u) +
+			     (2u * (uint32_t)(j % wpc) + 1u < n && (uint32_t)(v[u] >> 32) != 0u ? 1u : 0u);
+		}
+	}
 	for (int o = 32; o > 0; o >>= 1)
 		s += __shfl_xor(s, o);
 	/* a pop's decrement can land before its push's increment */

@@ -167,7 +167,7 @@ __device__ __forceinline__ uint32_t whole_superblocks(const kma_dheap *H, uint32
 					     uint32_t cap, bool want, const uint16_t *order, uint64_t *res)
 {
 	uint32_t sb = 0;
-	bool have = want && pool_pop(H, x, ctx + grp_lane(), &sb);
+	bool have = want && pool_pop(H, x, ctx, grp_lane(), &sb);
 	uint32_t old[KMA_WHOLE_WORDS];
 	const uint32_t W = (cap + 31u) >> 5;
 	const uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;

@@ -15,6 +15,7 @@
 /* the host mirror of the device header must match it byte for byte */
 static_assert(sizeof(kma_dheap) == sizeof(nat_hdr), "kma_dheap / nat_hdr size");
 static_assert(offsetof(kma_dheap, off_psum) == offsetof(nat_hdr, off_psum), "off_psum");
+static_assert(offsetof(kma_dheap, spare_slots) == offsetof(nat_hdr, spare_slots), "spare_slots");
 static_assert(offsetof(kma_dheap, class_magic) == offsetof(nat_hdr, class_magic), "class_magic");
 static_assert(KMA_STATS_BYTES == NAT_STATS_BYTES && KMA_PAGE_LEVELS == NAT_PAGE_LEVELS, "layout constants");
 
@@ -123,21 +124,13 @@ __global__ void k_trim(kma_dheap *H, uint32_t nctx, uint32_t nclasses)
 		try_release(H, sb, s, st_home(s), ctx);
 }
 
-/* After a kernel boundary every L2 has been written back, so superblocks in
- * dirty shards may go to any XCD: move them to their XCD's clean shards and
- * drop pending clean requests (one thread; no allocation may be in flight). */
+/* After a kernel boundary: drain dirty shards and spare lines into clean
+ * shards (kma_nat::clean_all, one thread). */
 __global__ void k_clean_dirty(kma_dheap *H)
 {
-	using namespace kma_nat;
 	if (blockIdx.x != 0 || threadIdx.x != 0)
 		return;
-	for (uint32_t x = 0; x < KMA_NXCD; x++) {
-		st(creq(H, x), (uint64_t)0);
-		uint32_t sb, n = 0;
-		for (uint32_t k = 0; k < dirty_per_xcd(H); k++)
-			while (pool_pop_shard(H, dirty_shard(H, x, k), &sb))
-				pool_push(H, clean_shard(H, x, n++), sb);
-	}
+	kma_nat::clean_all(H);
 }
 
 extern "C" int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses)

@@ -126,6 +126,14 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 	off = align_up(off + nh->npages, 256);
 	nh->off_psum = off;
 	off = align_up(off + NAT_PAGE_LEVELS * ((((nh->npages + 63) / 64) + 63) / 64) * 8, 256);
+	/* context spare lines: up to 16 empty superblocks per context, at most
+	 * a quarter of the superblocks over all contexts */
+	nh->off_spare = off;
+	off = align_up(off + (uint64_t)nh->nctx * 128, 256);
+	{
+		uint64_t q = nh->nsb / (4ull * nh->nctx);
+		nh->spare_slots = (uint32_t)(q < 16 ? q : 16);
+	}
 	/* metadata bytes; data placed after allocation so it is absolutely
 	 * aligned to the superblock and page sizes */
 	nh->off_data = off;

@@ -42,6 +42,9 @@ typedef struct nat_hdr {
 	uint64_t class_magic[32];
 	uint8_t class_for_log2[64];
 	uint64_t off_psum;
+	uint64_t off_spare;
+	uint32_t spare_slots;
+	uint32_t pad1;
 } nat_hdr;
 
 #define NAT_MAGIC 0x4e414d4b31414d4bull

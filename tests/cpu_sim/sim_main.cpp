@@ -44,12 +44,11 @@ static void trim(kma_dheap *H)
 			if (st_count(s) == 0 && st_cls1(s) != 0)
 				try_release(H, sb, s, st_home(s), ctx);
 		}
-	for (uint32_t x = 0; x < KMA_NXCD; x++) {
-		uint32_t sb, n = 0;
-		for (uint32_t k = 0; k < dirty_per_xcd(H); k++)
-			while (pool_pop_shard(H, dirty_shard(H, x, k), &sb))
-				pool_push(H, clean_shard(H, x, n++), sb);
-	}
+	clean_all(H);
+	for (uint32_t ctx = 0; ctx < H->nctx; ctx++)
+		for (uint32_t k = 0; k < 32; k++)
+			if (spare(H, ctx)[k])
+				fprintf(stderr, "ctx %u spare slot %u still holds %u\n", ctx, k, spare(H, ctx)[k]);
 }
 
 static int canonical_problems(kma_dheap *H)

@@ -11,7 +11,7 @@ HDR_DTYPE = np.dtype([
     ("off_pages", "<u8"), ("off_stats", "<u8"),
     ("class_size", "<u4", 32), ("class_cap", "<u4", 32), ("class_magic", "<u8", 32),
     ("class_for_log2", "u1", 64),
-    ("off_psum", "<u8"),
+    ("off_psum", "<u8"), ("off_spare", "<u8"), ("spare_slots", "<u4"), ("pad1", "<u4"),
 ])
 
 MAGIC = 0x4E414D4B31414D4B
@@ -55,6 +55,7 @@ class NatHeap:
             "stats": arr(h["off_stats"], 8, "<u8"),
             "pmask": arr(int(h["off_stats"]) + 256, 8 * 16, "<u8")[::16],
             "psum": arr(h["off_psum"], PAGE_LEVELS * ns, "<u8").reshape(PAGE_LEVELS, ns),
+            "spare": arr(h["off_spare"], self.nctx * 32, "<u4").reshape(self.nctx, 32),
             "state": arr(h["off_state"], self.nsb, "<u8"),
             "next": arr(h["off_next"], self.nsb, "<u4"),
             "bitmap": arr(h["off_bitmap"], self.nsb * self.bm_words, "<u4"),
@@ -84,9 +85,12 @@ class NatHeap:
     def canonical_problems(self):
         """A heap with no live block, after trim: every bitmap and page-bitmap
         word zero, every state word zero apart from its generation, every
-        context head empty, every superblock in exactly one pool shard."""
+        context head and spare line empty, every superblock in exactly one
+        pool shard."""
         m = self.meta()
         probs = []
+        if m["spare"].any():
+            probs.append("%d context spare slots hold superblocks" % int(np.count_nonzero(m["spare"])))
         if m["bitmap"].any():
             probs.append("%d bitmap words nonzero" % int(np.count_nonzero(m["bitmap"])))
         if m["pbitmap"].any():
