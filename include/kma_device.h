@@ -424,8 +424,8 @@ __device__ inline bool pool_pop_bits(const kma_dheap *H, uint32_t y, uint64_t bi
 }
 
 /* ---- context spares ----
- * Each context has a 128-B line of spare slots (u32 sb + 1, 0: empty; the
- * first spare_slots of them are used): empty superblocks the context keeps
+ * Each context has a 128-B line of 16 spare slots (u32 sb + 1, 0: empty;
+ * the first spare_slots of them are used): empty superblocks the context keeps
  * for its next refill, so that most superblock turnover touches only the
  * context's own line instead of the shared pool shards.  A spare is in the
  * same state as a pooled superblock (count 0, no class, generation bumped)
@@ -437,6 +437,7 @@ __device__ __forceinline__ uint32_t *spare(const kma_dheap *H, uint32_t ctx)
 {
 	return (uint32_t *)(base(H) + H->off_spare) + (uint64_t)ctx * 32u;
 }
+
 
 /* Keep empty superblock sb as a spare of context ctx: a CAS into slot
  * sb mod n, then into the slot half a line away; false if both are taken. */
@@ -553,7 +554,7 @@ __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint3
  * bounded time for one (serving requests aimed at its own XCD meanwhile);
  * requests already pending mean those XCDs have not answered, and the
  * caller fails at once. */
-#define KMA_CREQ_POLLS 16u
+#define KMA_CREQ_POLLS 4u
 #define KMA_STEAL_CTX 8u
 __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t salt, uint32_t *out)
 {
@@ -1007,35 +1008,57 @@ __host__ __device__ inline uint64_t init_items(uint64_t nsb, uint32_t npool, uin
 
 } /* namespace kma_nat */
 
-/* Superblocks currently free for any class: the sum of the shard counts
- * plus the occupied context spare slots, read by a full wavefront (all 64
- * lanes call it; every lane gets the sum).  A snapshot: pushes and pops in
- * flight may be missed. */
-__device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
+/* Superblocks currently free for any class, counted over slice `part` of
+ * `nparts` of the heap's free lists (shard counts, then context spare
+ * slots) by a full wavefront (all 64 lanes call it; every lane gets the
+ * slice's sum, modulo 2^64).  Summing the slices of parts 0 .. nparts-1,
+ * computed by as many wavefronts, gives the total at a fraction of one
+ * wavefront's time; a pop's decrement of a shard count can land before its
+ * push's increment, so read the total as signed and clamp it at zero.  A
+ * snapshot: pushes and pops in flight may be missed. */
+__device__ inline uint64_t kma_free_superblocks_part(const kma_dheap *H, uint32_t part, uint32_t nparts)
 {
 	using namespace kma_nat;
+	const uint32_t n = H->spare_slots, wpc = (n + 1u) / 2u; /* spare words per line */
+	const uint64_t ns = H->npool, N = ns + (uint64_t)H->nctx * wpc;
+	const uint64_t lo = N * part / nparts, hi = N * (part + 1u) / nparts;
 	uint64_t s = 0;
-	for (uint32_t k = lane(); k < H->npool; k += 64)
-		s += ld(pool(H, k) + 1);
-	const uint32_t n = H->spare_slots, wpc = (n + 1u) / 2u; /* words per line */
-	const uint64_t nw = (uint64_t)H->nctx * wpc;
-	for (uint64_t i = lane(); i < nw; i += 4u * 64u) {
+	for (uint64_t i = lo + lane(); i < hi; i += 4u * 64u) {
 		uint64_t v[4];
 #pragma unroll
 		for (uint32_t u = 0; u < 4u; u++) {
 			uint64_t j = i + u * 64u;
-			v[u] = j < nw ? ld((const uint64_t *)spare(H, (uint32_t)(j / wpc)) + j % wpc) : 0u;
+			if (j >= hi)
+				v[u] = 0u;
+			else if (j < ns)
+				v[u] = ld(pool(H, (uint32_t)j) + 1);
+			else
+				v[u] = ld((const uint64_t *)spare(H, (uint32_t)((j - ns) / wpc)) + (j - ns) % wpc);
 		}
 #pragma unroll
 		for (uint32_t u = 0; u < 4u; u++) {
 			uint64_t j = i + u * 64u;
-			s += ((uint32_t)v[u] != 0u ? 1u : 0u) +
-			     (2u * (uint32_t)(j % wpc) + 1u < n && (uint32_t)(v[u] >> 32) != 0u ? 1u : 0u);
+			if (j >= hi)
+				continue;
+			if (j < ns) {
+				s += v[u];
+			} else {
+				uint32_t w = (uint32_t)((j - ns) % wpc);
+				s += ((uint32_t)v[u] != 0u ? 1u : 0u) +
+				     (2u * w + 1u < n && (uint32_t)(v[u] >> 32) != 0u ? 1u : 0u);
+			}
 		}
 	}
 	for (int o = 32; o > 0; o >>= 1)
 		s += __shfl_xor(s, o);
-	/* a pop's decrement can land before its push's increment */
+	return s;
+}
+
+/* Superblocks currently free for any class (all of the heap's free lists,
+ * by one full wavefront; see kma_free_superblocks_part). */
+__device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
+{
+	uint64_t s = kma_free_superblocks_part(H, 0u, 1u);
 	return (int64_t)s < 0 ? 0u : s;
 }
 

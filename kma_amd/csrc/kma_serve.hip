@@ -63,6 +63,7 @@ enum : uint32_t {
 	C_RQ_HEAD = 193,
 	C_RQ_TAIL = 194,
 	C_SEQ = 195,
+	C_POOLED = 224,      /* free superblocks, summed over the work-groups' slices */
 	C_STEP_STALLS = 240, /* stalls of the current step (all work-groups) */
 };
 constexpr uint64_t PFLAG = 1ull << 63; /* done-list slot field: preempted, not completed */
@@ -255,17 +256,21 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 	uint64_t c_mallocs = 0, c_stalls = 0, c_frees = 0, c_bad = 0;
 	if (blockIdx.x == 0 && tid == 0)
 		ctl[C_T0] = wall_clock64();
+	/* every work-group counts its slice of the heap's free superblocks
+	 * (here, then after each step's frees); phase A reads the sum */
+	if (tid < 64) {
+		uint64_t part = kma_free_superblocks_part(H, blockIdx.x, nwg);
+		if (tid == 0)
+			fadd(ctl + C_POOLED, part);
+	}
 	if (!grid_sync(ctl, nwg))
 		return;
 	const uint64_t t0 = ctl[C_T0];
 	for (;;) {
 		/* ---- A: retire, admit, stop? ---- */
-		if (blockIdx.x == 0) {
-			uint64_t pooled = 0;
-			if (tid < 64)
-				pooled = kma_pooled_superblocks(H);
-			if (tid == 0)
-				admit(a, t0, pooled);
+		if (blockIdx.x == 0 && tid == 0) {
+			uint64_t pooled = xchg(ctl + C_POOLED, (uint64_t)0);
+			admit(a, t0, (int64_t)pooled < 0 ? 0u : pooled);
 		}
 		if (!grid_sync(ctl, nwg))
 			return;
@@ -340,6 +345,11 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 				c_frees++;
 			}
 			kma_wg_free<SV_BLOCK>(H, q, L);
+		}
+		if (tid < 64) {
+			uint64_t part = kma_free_superblocks_part(H, blockIdx.x, nwg);
+			if (tid == 0)
+				fadd(ctl + C_POOLED, part);
 		}
 		if (!grid_sync(ctl, nwg))
 			return;
