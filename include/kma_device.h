@@ -496,6 +496,7 @@ __device__ inline bool spare_take(const kma_dheap *H, uint32_t ctx, uint32_t occ
  * remote freer) before its push, which the pop observed, so the one
  * write-back covers the whole batch. */
 #define KMA_MIGRATE_MAX 64u
+#define KMA_MIGRATE_CTX 32u
 __device__ inline void migrate_dirty(const kma_dheap *H, uint32_t x, uint32_t ctx)
 {
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
@@ -513,13 +514,17 @@ __device__ inline void migrate_dirty(const kma_dheap *H, uint32_t x, uint32_t ct
 			n++;
 		}
 	}
-	/* then the spares of x's contexts */
+	/* then the spares of up to KMA_MIGRATE_CTX of x's contexts (a rare
+	 * path: one line at a time keeps the callers' register budget) */
 	uint32_t per = H->nctx / KMA_NXCD;
-	for (uint32_t i = 0; i < per && n < KMA_MIGRATE_MAX && H->spare_slots; i++) {
+	for (uint32_t i = 0; i < per && i < KMA_MIGRATE_CTX && n < KMA_MIGRATE_MAX && H->spare_slots; i++) {
 		uint32_t c2 = x * per + (ctx + i) % per;
-		uint32_t occ = spare_occupied(H, c2);
-		while (n < KMA_MIGRATE_MAX && spare_take(H, c2, occ, 0u, &sb)) {
-			occ = spare_occupied(H, c2);
+		uint32_t *sp = spare(H, c2);
+		for (uint32_t m = spare_occupied(H, c2); m && n < KMA_MIGRATE_MAX; m &= m - 1u) {
+			uint32_t v = xchg(sp + (__ffs(m) - 1u), 0u);
+			if (!v)
+				continue;
+			sb = v - 1u;
 			xchg(pnext(H, sb), first);
 			if (n == 0)
 				last = sb;
