@@ -40,6 +40,33 @@ COPY_BYTES = 1 << 30  # the copy probe: reads and writes 1 GiB
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_full_occupancy_batched.md")
 
 
+L2_PROFILE = os.path.join(ROOT, "profiles", "l2_requests_full_occupancy_batched.md")
+
+
+def l2_requests_per_round():
+    """L2 (TCC) requests per round from the committed counter profile, or None."""
+    try:
+        for line in open(L2_PROFILE):
+            if line.startswith("tcc_requests_per_round:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def l2_roofline(ms_per_round, store_peak):
+    """L2 request rate of the batched launch (counter-measured requests per
+    round over the measured time per round) against the scattered-store
+    rate of the same shape."""
+    req = l2_requests_per_round()
+    if not req:
+        return None
+    rate = req / (ms_per_round * 1e-3)
+    return {"requests_per_round": req, "achieved_requests_per_s": rate,
+            "peak_scattered_4b_stores_per_s": store_peak, "frac": rate / store_peak,
+            "source": "profiles/l2_requests_full_occupancy_batched.md"}
+
+
 def traffic_per_round():
     """Bytes per round from the committed counter profile, or None."""
     try:
@@ -244,6 +271,8 @@ def main():
         ops, _ = kma_amd.atomic_peak(1 << 30, NWI, 64, pat)
         peak[name] = ops
         log("bench: atomicOr peak (%s) %.3e /s" % (name, ops))
+    store_peak, _ = kma_amd.atomic_peak(1 << 30, NWI, 64, 3)
+    log("bench: scattered 4-B store peak %.3e /s" % store_peak)
 
     b, p = res["batched"], res["plain"]
     value = b["ops_per_s"]
@@ -285,6 +314,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": (traffic_per_round() * K) if traffic_per_round() else None,
         },
+        "l2_request_roofline": l2_roofline(b["gpu_ms"] / K, store_peak),
         "atomic_roofline": {
             "peak_distinct_atomics_per_s": peak["distinct"],
             "peak_same_word_per_wave_atomics_per_s": peak["same_word_per_wave"],

@@ -559,7 +559,7 @@ __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint3
  * bounded time for one (serving requests aimed at its own XCD meanwhile);
  * requests already pending mean those XCDs have not answered, and the
  * caller fails at once. */
-#define KMA_CREQ_POLLS 4u
+#define KMA_CREQ_POLLS 1u
 #define KMA_STEAL_CTX 8u
 __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t salt, uint32_t *out)
 {

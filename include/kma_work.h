@@ -82,9 +82,11 @@ int kma_trace_serial(kma_heap *h, const int64_t *ops, uint64_t n, uint64_t *out,
 
 /* Agent-scope atomicOr throughput over a `bytes`-byte device buffer:
  * pattern 0 distinct pseudo-random words, 1 coalesced consecutive words,
- * 2 one word per wavefront.  nthreads lanes x iters atomics; the first
- * `warmup` launches are untimed.  Returns atomics per second in *ops_per_s
- * and the average launch time in *ms. */
+ * 2 one word per wavefront; pattern 3 instead issues plain 4-B stores,
+ * `iters` consecutive words per lane into the lane's own 2 KiB slot (the
+ * store shape of the full-occupancy workload).  nthreads lanes x iters
+ * operations; the first `warmup` launches are untimed.  Returns operations
+ * per second in *ops_per_s and the average launch time in *ms. */
 int kma_atomic_peak(uint64_t bytes, uint64_t nthreads, uint32_t iters, int pattern,
 		    uint32_t warmup, uint32_t reps, double *ops_per_s, double *ms);
 

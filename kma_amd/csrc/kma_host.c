@@ -455,7 +455,7 @@ done:
 int kma_atomic_peak(uint64_t bytes, uint64_t nthreads, uint32_t iters, int pattern, uint32_t warmup,
 		    uint32_t reps, double *ops_per_s, double *ms_out)
 {
-	if (bytes < 4096 || nthreads == 0 || iters == 0 || reps == 0 || pattern < 0 || pattern > 2) {
+	if (bytes < 4096 || nthreads == 0 || iters == 0 || reps == 0 || pattern < 0 || pattern > 3) {
 		set_err("bad arguments");
 		return -1;
 	}

@@ -216,12 +216,22 @@ extern "C" int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nro
  * use.  pattern 0: every lane hits a pseudo-random distinct 4-B word over
  * the whole buffer (distinct-address peak); 1: lanes of a wavefront hit
  * consecutive words (coalesced); 2: all lanes of a wavefront hit one word
- * (contended).  Each lane issues `iters` atomics.
+ * (contended).  Each lane issues `iters` atomics.  Pattern 3 is the L2
+ * request-rate reference for the full-occupancy workload's hold phase:
+ * plain 4-B stores, each lane writing `iters` consecutive words of its own
+ * 2 KiB slot at a pseudo-random position in the buffer.
  */
 __global__ void __launch_bounds__(256) k_atomic_peak(uint32_t *buf, uint64_t nwords, uint32_t iters, int pattern)
 {
 	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
+	if (pattern == 3) {
+		uint64_t slot = (kma_mix(7, gid, 0, 0) >> 8) % (nwords / 512u);
+		uint32_t *p = buf + slot * 512u;
+		for (uint32_t i = 0; i < iters; i++)
+			wl_store(p + i % 511u, i);
+		return;
+	}
 	uint32_t acc = 0;
 	for (uint32_t i = 0; i < iters; i++) {
 		uint64_t w;
