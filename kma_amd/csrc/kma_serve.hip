@@ -100,7 +100,7 @@ __device__ __forceinline__ uint64_t pattern_id(const kma_serve_args &a, uint64_t
 }
 
 /* Grid barrier for a co-resident grid; false if the loop was aborted. */
-__device__ bool grid_sync(uint64_t *ctl, uint32_t nwg)
+__device__ __forceinline__ bool grid_sync(uint64_t *ctl, uint32_t nwg)
 {
 	using namespace kma_nat;
 	__shared__ int s_ok;
@@ -144,7 +144,7 @@ __device__ __forceinline__ bool lentry_live(const kma_serve_args &a, uint64_t e)
 }
 
 /* Phase A, one lane. */
-__device__ void admit(const kma_serve_args &a, uint64_t t0, uint64_t pooled)
+__device__ __forceinline__ void admit(const kma_serve_args &a, uint64_t t0, uint64_t pooled)
 {
 	const kma_dheap *H = (const kma_dheap *)a.heap;
 	uint64_t *ctl = a.ctl;

@@ -72,7 +72,7 @@ struct phases {
 };
 
 /* all 64 lanes of the wavefront are active when this runs */
-__device__ void flush(uint64_t *ctl, const counters &c, const phases &ph)
+__device__ __forceinline__ void flush(uint64_t *ctl, const counters &c, const phases &ph)
 {
 	for (int k = 0; k < C_N; k++) {
 		uint64_t x = c.v[k];
@@ -101,7 +101,7 @@ __device__ __forceinline__ void count_malloc(const kma_dheap *H, uint32_t size, 
 /* The block of request t's malloc: waits for the result slot to carry this
  * run's epoch, checks the pattern.  NULL if the malloc failed (or the wait
  * gave up, which is counted). */
-__device__ void *take_target(const kma_dheap *H, uint64_t *res, uint32_t t, uint32_t ep, counters &c)
+__device__ __forceinline__ void *take_target(const kma_dheap *H, uint64_t *res, uint32_t t, uint32_t ep, counters &c)
 {
 	using namespace kma_nat;
 	uint64_t v = ld(res + t);
