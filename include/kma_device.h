@@ -806,9 +806,10 @@ __device__ __forceinline__ uint64_t page_windows(uint64_t w, uint32_t l, uint64_
 	uint64_t z = ~w & valid;
 	for (uint32_t sh = 1; sh < (1u << l); sh <<= 1)
 		z &= z >> sh;
-	uint64_t pat = 0;
-	for (uint32_t i = 0; i < 64; i += 1u << l)
-		pat |= 1ull << i;
+	/* window starts: every multiple of 2^l, by doubling */
+	uint64_t pat = 1;
+	for (uint32_t sh = 1u << l; sh < 64u; sh <<= 1)
+		pat |= pat << sh;
 	return z & pat;
 }
 
@@ -834,13 +835,24 @@ __device__ __forceinline__ void psum_gained(const kma_dheap *H, uint64_t wi, uin
 			f_and(psum(H, l) + (wi >> 6), ~((uint64_t)1 << (wi & 63)));
 }
 
-/* Set the hint of every level that lost its last window. */
+/* Set the hint of every level that lost its last window: the protocol of
+ * psum_mark_full, with one wait and one re-read for all the levels. */
 __device__ __forceinline__ void psum_lost(const kma_dheap *H, uint64_t wi, uint64_t before, uint64_t after)
 {
 	uint64_t valid = page_valid(H, wi);
+	uint64_t bit = 1ull << (wi & 63);
+	uint32_t set = 0;
 	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
-		if (page_windows(before, l, valid) && !page_windows(after, l, valid))
-			psum_mark_full(H, l, wi);
+		if (page_windows(before, l, valid) && !page_windows(after, l, valid) &&
+		    !(f_or(psum(H, l) + (wi >> 6), bit) & bit))
+			set |= 1u << l;
+	if (!set)
+		return;
+	order_prior();
+	uint64_t w = f_or(pbits(H) + wi, (uint64_t)0);
+	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
+		if (((set >> l) & 1u) && page_windows(w, l, valid))
+			f_and(psum(H, l) + (wi >> 6), ~bit);
 }
 
 __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t seed)
