@@ -92,7 +92,7 @@ typedef struct kma_dheap {
 	uint32_t class_cap[32];
 	uint64_t class_magic[32]; /* ceil(2^40 / size): exact block index */
 	uint8_t class_for_log2[64];
-	uint64_t off_psum;     /* page-window summaries, KMA_PAGE_LEVELS levels */
+	uint64_t off_psum;     /* page-window summaries: KMA_PAGE_LEVELS levels, then their top level */
 	uint64_t off_spare;    /* context spare lines, 128 B per context */
 	uint32_t spare_slots;  /* slots used per spare line (0..16) */
 	uint32_t pad1;
@@ -255,6 +255,8 @@ __device__ __forceinline__ uint64_t *pmask(const kma_dheap *H, uint32_t x) { ret
 __device__ __forceinline__ uint64_t *creq(const kma_dheap *H, uint32_t x) { return pmask(H, x) + 1; }
 __device__ __forceinline__ uint64_t psum_words(const kma_dheap *H) { return (((H->npages + 63) >> 6) + 63) >> 6; }
 __device__ __forceinline__ uint64_t *psum(const kma_dheap *H, uint32_t l) { return (uint64_t *)(base(H) + H->off_psum) + l * psum_words(H); }
+__device__ __forceinline__ uint64_t psum2_words(const kma_dheap *H) { return (psum_words(H) + 63) >> 6; }
+__device__ __forceinline__ uint64_t *psum2(const kma_dheap *H, uint32_t l) { return psum(H, KMA_PAGE_LEVELS) + l * psum2_words(H); }
 
 __device__ __forceinline__ void bump_stat(const kma_dheap *H, int k)
 {
@@ -791,9 +793,14 @@ __device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint3
  *     that gained a window, after its own bitmap update completed.
  * Whichever of a racing set and clear comes last, one of the two sides sees
  * the other's bitmap update, so a window is never hidden once both finish.
- * An allocation reads the summary words (coalesced across the wavefront)
- * and visits only words that may hold a window, so a full page region fails
- * after psum_words() reads. */
+ * A second level, psum2(H, l), holds one bit per summary word, set when all
+ * of its bits are, with the same protocol one level up (a setter that made
+ * a summary word all ones sets its bit, waits, re-reads the summary word and
+ * clears the bit again if it is no longer full; a clear that took a summary
+ * word out of all ones clears its bit).  An allocation reads the top-level
+ * words, then only summary words that may have a candidate, then only
+ * bitmap words that may hold a window: a full page region fails after
+ * psum2_words() reads. */
 __device__ __forceinline__ uint64_t page_valid(const kma_dheap *H, uint64_t wi)
 {
 	uint64_t n = H->npages - (wi << 6);
@@ -813,16 +820,40 @@ __device__ __forceinline__ uint64_t page_windows(uint64_t w, uint32_t l, uint64_
 	return z & pat;
 }
 
+/* Summary word si of level l went all ones (the caller's set did it): set
+ * its top-level bit, wait, and clear it again if the word is no longer full. */
+__device__ __forceinline__ void psum2_full(const kma_dheap *H, uint32_t l, uint64_t si)
+{
+	uint64_t *p2 = psum2(H, l) + (si >> 6);
+	uint64_t b2 = 1ull << (si & 63);
+	if (f_or(p2, b2) & b2)
+		return;
+	order_prior();
+	if (f_or(psum(H, l) + si, (uint64_t)0) != ~0ull)
+		f_and(p2, ~b2);
+}
+
+/* A summary bit of word si (level l) was cleared; `old` is the word before:
+ * if it was all ones, clear its top-level bit. */
+__device__ __forceinline__ void psum2_gained(const kma_dheap *H, uint32_t l, uint64_t si, uint64_t old)
+{
+	if (old == ~0ull)
+		f_and(psum2(H, l) + (si >> 6), ~((uint64_t)1 << (si & 63)));
+}
+
 __device__ __forceinline__ void psum_mark_full(const kma_dheap *H, uint32_t l, uint64_t wi)
 {
 	uint64_t *sp = psum(H, l) + (wi >> 6);
 	uint64_t bit = 1ull << (wi & 63);
-	if (f_or(sp, bit) & bit)
+	uint64_t old = f_or(sp, bit);
+	if (old & bit)
 		return;
 	order_prior();
 	uint64_t w = f_or(pbits(H) + wi, (uint64_t)0);
 	if (page_windows(w, l, page_valid(H, wi)))
-		f_and(sp, ~bit);
+		psum2_gained(H, l, wi >> 6, f_and(sp, ~bit));
+	else if ((old | bit) == ~0ull)
+		psum2_full(H, l, wi >> 6);
 }
 
 /* Clear the hint of every level that gained a window going from `before` to
@@ -832,7 +863,7 @@ __device__ __forceinline__ void psum_gained(const kma_dheap *H, uint64_t wi, uin
 	uint64_t valid = page_valid(H, wi);
 	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
 		if (!page_windows(before, l, valid) && page_windows(after, l, valid))
-			f_and(psum(H, l) + (wi >> 6), ~((uint64_t)1 << (wi & 63)));
+			psum2_gained(H, l, wi >> 6, f_and(psum(H, l) + (wi >> 6), ~((uint64_t)1 << (wi & 63))));
 }
 
 /* Set the hint of every level that lost its last window: the protocol of
@@ -841,21 +872,31 @@ __device__ __forceinline__ void psum_lost(const kma_dheap *H, uint64_t wi, uint6
 {
 	uint64_t valid = page_valid(H, wi);
 	uint64_t bit = 1ull << (wi & 63);
-	uint32_t set = 0;
-	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
-		if (page_windows(before, l, valid) && !page_windows(after, l, valid) &&
-		    !(f_or(psum(H, l) + (wi >> 6), bit) & bit))
+	uint32_t set = 0, full = 0;
+	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++) {
+		if (!page_windows(before, l, valid) || page_windows(after, l, valid))
+			continue;
+		uint64_t old = f_or(psum(H, l) + (wi >> 6), bit);
+		if (!(old & bit)) {
 			set |= 1u << l;
+			full |= (old | bit) == ~0ull ? 1u << l : 0u;
+		}
+	}
 	if (!set)
 		return;
 	order_prior();
 	uint64_t w = f_or(pbits(H) + wi, (uint64_t)0);
-	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
-		if (((set >> l) & 1u) && page_windows(w, l, valid))
-			f_and(psum(H, l) + (wi >> 6), ~bit);
+	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++) {
+		if (!((set >> l) & 1u))
+			continue;
+		if (page_windows(w, l, valid))
+			psum2_gained(H, l, wi >> 6, f_and(psum(H, l) + (wi >> 6), ~bit));
+		else if ((full >> l) & 1u)
+			psum2_full(H, l, wi >> 6);
+	}
 }
 
-__device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t seed)
+__device__ __forceinline__ void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t seed)
 {
 	uint64_t psz = 1ull << H->page_log2;
 	uint64_t k = (size + psz - 1) >> H->page_log2;
@@ -865,59 +906,71 @@ __device__ inline void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t s
 	while ((1ull << l) < k)
 		l++;
 	uint64_t run = (k == 64) ? ~0ull : ((1ull << k) - 1ull);
-	uint64_t ns = psum_words(H);
-	uint64_t *sp = psum(H, l);
+	const uint64_t ns = psum_words(H), ns2 = psum2_words(H);
+	uint64_t *sp = psum(H, l), *sp2 = psum2(H, l);
 	uint64_t *pb = pbits(H);
 	uint32_t h = seed * 0x9E3779B1u;
-	uint64_t si = (uint64_t)(h >> 8) % ns;
-	uint32_t rot = h & 63u;
+	const uint64_t si0 = (uint64_t)(h >> 8) % ns;
+	const uint32_t rot = h & 63u, rot2 = (uint32_t)(si0 & 63u);
 	waiter wt;
-	for (uint64_t pass = 0; pass < ns; pass++, si = (si + 1 == ns) ? 0 : si + 1) {
-		uint64_t cand = ~ld(sp + si);
-		while (cand) {
-			if (wt.expired()) {
-				bump_stat(H, KMA_STAT_WAIT_POOL);
-				return nullptr;
-			}
-			/* start at a seed-dependent bit so callers spread out */
-			uint64_t r = (cand >> rot) | (rot ? cand << (64u - rot) : 0ull);
-			uint32_t b = ((uint32_t)__ffsll((unsigned long long)r) - 1u + rot) & 63u;
-			uint64_t wi = (si << 6) + b;
-			if (wi >= ((H->npages + 63) >> 6)) { /* no such word */
-				cand &= ~(1ull << b);
+	for (uint64_t jp = 0; jp < ns2; jp++) {
+		uint64_t t = (si0 >> 6) + jp;
+		t = t >= ns2 ? t - ns2 : t;
+		uint64_t c2 = ~ld(sp2 + t);
+		while (c2) {
+			/* summary words from a seed-dependent one on */
+			uint64_t r2 = (c2 >> rot2) | (rot2 ? c2 << (64u - rot2) : 0ull);
+			uint32_t b2 = ((uint32_t)__ffsll((unsigned long long)r2) - 1u + rot2) & 63u;
+			c2 &= ~(1ull << b2);
+			const uint64_t si = (t << 6) + b2;
+			if (si >= ns)
 				continue;
-			}
-			uint64_t valid = page_valid(H, wi);
-			uint64_t w = ld(pb + wi);
-			for (;;) {
+			uint64_t cand = ~ld(sp + si);
+			while (cand) {
 				if (wt.expired()) {
 					bump_stat(H, KMA_STAT_WAIT_POOL);
 					return nullptr;
 				}
-				uint64_t win = page_windows(w, l, valid);
-				if (!win) {
-					psum_mark_full(H, l, wi);
+				/* start at a seed-dependent bit so callers spread out */
+				uint64_t r = (cand >> rot) | (rot ? cand << (64u - rot) : 0ull);
+				uint32_t b = ((uint32_t)__ffsll((unsigned long long)r) - 1u + rot) & 63u;
+				uint64_t wi = (si << 6) + b;
+				if (wi >= ((H->npages + 63) >> 6)) { /* no such word */
 					cand &= ~(1ull << b);
-					break;
+					continue;
 				}
-				uint32_t j = (uint32_t)__ffsll((unsigned long long)win) - 1u;
-				uint64_t m = run << j;
-				uint64_t old = f_or(pb + wi, m);
-				if ((old & m) == 0) {
-					uint64_t pi = (wi << 6) + j;
-					st(plen(H) + pi, (uint8_t)k);
-					psum_lost(H, wi, old, old | m);
-					return base(H) + H->off_pages + (pi << H->page_log2);
-				}
-				/* lost a race: give back the pages this call took */
-				uint64_t mine = m & ~old;
-				if (mine) {
-					uint64_t o2 = f_and(pb + wi, ~mine);
-					order_prior();
-					psum_gained(H, wi, o2, o2 & ~mine);
-					w = o2 & ~mine;
-				} else {
-					w = old;
+				uint64_t valid = page_valid(H, wi);
+				uint64_t w = ld(pb + wi);
+				for (;;) {
+					if (wt.expired()) {
+						bump_stat(H, KMA_STAT_WAIT_POOL);
+						return nullptr;
+					}
+					uint64_t win = page_windows(w, l, valid);
+					if (!win) {
+						psum_mark_full(H, l, wi);
+						cand &= ~(1ull << b);
+						break;
+					}
+					uint32_t j = (uint32_t)__ffsll((unsigned long long)win) - 1u;
+					uint64_t m = run << j;
+					uint64_t old = f_or(pb + wi, m);
+					if ((old & m) == 0) {
+						uint64_t pi = (wi << 6) + j;
+						st(plen(H) + pi, (uint8_t)k);
+						psum_lost(H, wi, old, old | m);
+						return base(H) + H->off_pages + (pi << H->page_log2);
+					}
+					/* lost a race: give back the pages this call took */
+					uint64_t mine = m & ~old;
+					if (mine) {
+						uint64_t o2 = f_and(pb + wi, ~mine);
+						order_prior();
+						psum_gained(H, wi, o2, o2 & ~mine);
+						w = o2 & ~mine;
+					} else {
+						w = old;
+					}
 				}
 			}
 		}
@@ -977,10 +1030,25 @@ __device__ inline void clean_all(const kma_dheap *H)
  *     j + C, j + 2C, ... (lowest on top); dirty shards start empty;
  *   - each XCD's shard mask has the bits of its non-empty clean shards;
  *   - a page-window summary bit starts set only for a word that does not
- *     exist or cannot hold a window of that level (a partial last word). */
+ *     exist or cannot hold a window of that level (a partial last word), a
+ *     top-level bit only for a summary word that does not exist or is all
+ *     ones. */
 __device__ __forceinline__ uint64_t init_shard_len(uint64_t j, uint64_t nsb, uint64_t C)
 {
 	return j < nsb ? (nsb - j + C - 1u) / C : 0u;
+}
+
+/* Summary word si of level l in the empty page region: a bit set only for a
+ * bitmap word that does not exist or cannot hold a window of the level. */
+__device__ inline uint64_t init_psum_word(const kma_dheap *H, uint32_t l, uint64_t si)
+{
+	uint64_t nwords = (H->npages + 63) >> 6, bits = 0;
+	for (uint32_t b = 0; b < 64; b++) {
+		uint64_t wi = (si << 6) + b;
+		if (wi >= nwords || !page_windows(0, l, page_valid(H, wi)))
+			bits |= (uint64_t)1 << b;
+	}
+	return bits;
 }
 
 __device__ inline void init_item(const kma_dheap *H, uint64_t i)
@@ -1003,16 +1071,21 @@ __device__ inline void init_item(const kma_dheap *H, uint64_t i)
 				m |= (uint64_t)1 << (D + (j - i * K));
 		*pmask(H, (uint32_t)i) = m;
 	}
-	uint64_t ns = psum_words(H), nwords = (H->npages + 63) >> 6;
+	uint64_t ns = psum_words(H), ns2 = psum2_words(H);
 	if (H->npages && i < KMA_PAGE_LEVELS * ns) {
 		uint32_t l = (uint32_t)(i / ns);
-		uint64_t si = i % ns, bits = 0;
+		uint64_t si = i % ns;
+		psum(H, l)[si] = init_psum_word(H, l, si);
+	}
+	if (H->npages && i < KMA_PAGE_LEVELS * ns2) {
+		uint32_t l = (uint32_t)(i / ns2);
+		uint64_t j = i % ns2, bits = 0;
 		for (uint32_t b = 0; b < 64; b++) {
-			uint64_t wi = (si << 6) + b;
-			if (wi >= nwords || !page_windows(0, l, page_valid(H, wi)))
+			uint64_t si = (j << 6) + b;
+			if (si >= ns || init_psum_word(H, l, si) == ~0ull)
 				bits |= (uint64_t)1 << b;
 		}
-		psum(H, l)[si] = bits;
+		psum2(H, l)[j] = bits;
 	}
 }
 

@@ -184,10 +184,10 @@ extern "C" int kmad_serial(int layout, void *heap, const int64_t *d_ops, uint64_
 /* ------------------------------------------------------------------ */
 /* plain per-launch workloads                                           */
 /* ------------------------------------------------------------------ */
-template <class HeapT>
+template <class HeapT, bool GENERIC>
 __global__ void __launch_bounds__(256) k_rounds(HeapT *heap, kma_wl_args a)
 {
-	wl_rounds<HeapT>(heap, a);
+	wl_rounds<HeapT, GENERIC>(heap, a);
 }
 
 extern "C" int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed,
@@ -199,11 +199,17 @@ extern "C" int kmad_rounds(int layout, void *heap, uint32_t round0, uint32_t nro
 	uint64_t blocks = (nwi + block - 1) / block;
 	hipStream_t s = (hipStream_t)stream;
 	kma_wl_args a = {round0, nrounds, mode, fixed, seed, nwi, variant, d_stats, d_lat};
-	if (layout == 0)
-		hipLaunchKernelGGL(k_rounds<clheap>, dim3((uint32_t)blocks), dim3(block), 0, s, (clheap *)heap, a);
+	const bool generic = variant != 0 || d_lat != nullptr;
+	if (layout == 0 && generic)
+		hipLaunchKernelGGL((k_rounds<clheap, true>), dim3((uint32_t)blocks), dim3(block), 0, s, (clheap *)heap, a);
+	else if (layout == 0)
+		hipLaunchKernelGGL((k_rounds<clheap, false>), dim3((uint32_t)blocks), dim3(block), 0, s, (clheap *)heap, a);
+	else if (generic)
+		hipLaunchKernelGGL((k_rounds<kma_dheap, true>), dim3((uint32_t)blocks), dim3(block), 0, s,
+				   (kma_dheap *)heap, a);
 	else
-		hipLaunchKernelGGL(k_rounds<kma_dheap>, dim3((uint32_t)blocks), dim3(block), 0, s, (kma_dheap *)heap,
-				   a);
+		hipLaunchKernelGGL((k_rounds<kma_dheap, false>), dim3((uint32_t)blocks), dim3(block), 0, s,
+				   (kma_dheap *)heap, a);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }

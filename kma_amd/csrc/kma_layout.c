@@ -125,7 +125,11 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 	nh->off_plen = off;
 	off = align_up(off + nh->npages, 256);
 	nh->off_psum = off;
-	off = align_up(off + NAT_PAGE_LEVELS * ((((nh->npages + 63) / 64) + 63) / 64) * 8, 256);
+	{
+		/* summary words per level, then their top-level words per level */
+		uint64_t ns = (((nh->npages + 63) / 64) + 63) / 64, ns2 = (ns + 63) / 64;
+		off = align_up(off + NAT_PAGE_LEVELS * (ns + ns2) * 8, 256);
+	}
 	/* context spare lines: up to 16 empty superblocks per context, at most
 	 * a quarter of the superblocks over all contexts */
 	nh->off_spare = off;

@@ -116,17 +116,20 @@ struct kma_phase {
 	}
 };
 
-template <class HeapT>
+/* GENERIC = false: variant 0 and no latency probes compiled in (see
+ * wl_batched). */
+template <class HeapT, bool GENERIC>
 __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 {
+	uint32_t *const lat = GENERIC ? a.lat : nullptr;
 	const uint32_t round0 = a.round0, nrounds = a.nrounds, fixed = a.fixed;
 	const int mode = a.mode;
 	const uint64_t seed = a.seed, nwi = a.nwi;
 	unsigned long long *stats = a.stats;
 	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
+	uint32_t ops = 0, nulls = 0, bad = 0, mallocs = 0; /* per work-item: a few per round */
 	bool alive = gid < nwi;
-	kma_phase ph((a.variant & KMA_VAR_PHASES) != 0);
+	kma_phase ph(GENERIC && (a.variant & KMA_VAR_PHASES) != 0);
 	for (uint32_t r = round0; r < round0 + nrounds; r++) {
 		if (!alive)
 			break;
@@ -139,10 +142,10 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 				hold = (uint32_t)(kma_mix(seed, gid, r, 1) % 65);
 			}
 			ph.start();
-			uint64_t t0 = a.lat ? wall_clock64() : 0;
+			uint64_t t0 = lat ? wall_clock64() : 0;
 			uint32_t *p = (uint32_t *)kma_malloc(heap, size);
-			if (a.lat)
-				a.lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
+			if (lat)
+				lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
 			ph.mark(0);
 			mallocs++;
 			ops++;
@@ -233,7 +236,7 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 	kma_phase ph((variant & KMA_VAR_PHASES) != 0);
 	uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	bool active = gid < nwi;
-	uint64_t ops = 0, nulls = 0, bad = 0, mallocs = 0;
+	uint32_t ops = 0, nulls = 0, bad = 0, mallocs = 0; /* per work-item: two per round */
 	for (uint32_t r = round0; r < round0 + nrounds; r++) {
 		uint32_t size = 0, hold = 0;
 		if (active) {

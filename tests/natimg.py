@@ -55,6 +55,8 @@ class NatHeap:
             "stats": arr(h["off_stats"], 8, "<u8"),
             "pmask": arr(int(h["off_stats"]) + 256, 8 * 16, "<u8")[::16],
             "psum": arr(h["off_psum"], PAGE_LEVELS * ns, "<u8").reshape(PAGE_LEVELS, ns),
+            "psum2": arr(int(h["off_psum"]) + PAGE_LEVELS * ns * 8, PAGE_LEVELS * ((ns + 63) // 64),
+                         "<u8").reshape(PAGE_LEVELS, (ns + 63) // 64),
             "spare": arr(h["off_spare"], self.nctx * 32, "<u4").reshape(self.nctx, 32),
             "state": arr(h["off_state"], self.nsb, "<u8"),
             "next": arr(h["off_next"], self.nsb, "<u4"),
@@ -111,7 +113,22 @@ class NatHeap:
                 probs.append("shard %d non-empty but its hint bit is clear" % sh)
         if self.npages and not np.array_equal(m["psum"], self.psum_initial()):
             probs.append("page-window summaries differ from the empty-heap image")
+        if self.npages and not np.array_equal(m["psum2"], self.psum2_initial()):
+            probs.append("top-level page-window summaries differ from the empty-heap image")
         return probs
+
+    def psum2_initial(self):
+        """Top-level summary bits of an empty page region: set for summary
+        words that do not exist or are all ones."""
+        ps = self.psum_initial()
+        ns = ps.shape[1]
+        ns2 = (ns + 63) // 64
+        out = np.zeros((PAGE_LEVELS, ns2), dtype=np.uint64)
+        for lv in range(PAGE_LEVELS):
+            for si in range(ns2 * 64):
+                if si >= ns or int(ps[lv, si]) == (1 << 64) - 1:
+                    out[lv, si // 64] |= np.uint64(1 << (si % 64))
+        return out
 
     def psum_initial(self):
         """Page-window summary bits of an empty page region: set only for
