@@ -49,6 +49,29 @@ def test_serial_trace_invariants(seed):
         assert int(nh.meta()["stats"][0]) == 0  # no double free
 
 
+def _fill_and_free(h, size, n_try):
+    """One serialized trace: n_try mallocs of `size`, then a free of every
+    success; returns the number of successes."""
+    out = h.trace_serial([size] * n_try + [-(i + 1) for i in range(n_try)])
+    return int(np.count_nonzero(out[:n_try]))
+
+
+def test_page_region_fills_exactly():
+    """A 20,000-page region (five summary words per window level, so the
+    top-level summary decides which summary words are read): runs of one
+    size fill it exactly, i.e. every aligned window is found and none is
+    handed out twice (1-page runs: every page; 2 and 4: every aligned pair
+    and quad, the 32-page last word included; 64: the 312 full words), and
+    it comes back to the empty image."""
+    P = 20000
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16, nctx=8, npages=P) as h:
+        for pages, expect in ((1, P), (2, P // 2), (4, P // 4), (64, P // 64), (1, P)):
+            _say("page fill: %d-page runs" % pages)
+            assert _fill_and_free(h, pages << 14, expect + 1) == expect, pages
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
 def test_serial_size_zero_and_oversize():
     with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=8, npages=64) as h:
         out = h.trace_serial([0, 8, -2, -1, (64 << 14) + 1, 1 << 20, -6])

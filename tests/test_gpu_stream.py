@@ -144,6 +144,25 @@ def test_stream_oversubscribed(form):
         assert nh.canonical_problems() == []
 
 
+def test_stream_large_page_region():
+    """Page runs from a 20,000-page region (several summary words per level,
+    the top-level summary in use) under the persistent form: invariants and
+    the empty-heap image afterwards."""
+    print("stream large page region", flush=True)
+    with stream_heap(sblocks=512, npages=20000) as h, kma_amd.Stream(48, 4096, seed=13) as s:
+        st = s.run(h, form=kma_amd.FORM_BATCHED, nwg=128, per_thread=2, variant=kma_amd.VAR_LOG)
+        check_run(st, s.info)
+        assert st["large"] > 0, st
+        req, _ = s.trace()
+        res, ticks = s.results(), s.times()
+        nh = NatHeap(h)
+        mal = (req >> 31) == 0
+        assert not block_problems(nh, res[mal], req[mal])
+        assert not lifetime_overlaps(req, res, ticks, nh)
+        h.trim()
+        assert nh.canonical_problems() == []
+
+
 def test_stream_replays_return_heap_to_empty():
     """Warm-up plus timed replays of the graph: each replay starts from the
     state the previous draining replay left."""
