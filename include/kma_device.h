@@ -104,7 +104,7 @@ typedef struct kma_dheap {
 #define KMA_STAT_POOL_EMPTY 2
 #define KMA_STAT_WAIT_RESERVE 3 /* a reserve() wait hit the watchdog */
 #define KMA_STAT_WAIT_CLAIM 4   /* a per-lane bitmap claim hit the watchdog */
-#define KMA_STAT_WAIT_WGCLAIM 5 /* a wavefront bitmap claim hit the watchdog */
+#define KMA_STAT_WAIT_WGCLAIM 5 /* a group claim hit the watchdog, or a whole-superblock claim lost bits */
 #define KMA_STAT_WAIT_POOL 6    /* a pool push/pop hit the watchdog */
 #define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
 #define KMA_STAT_MIGRATED 8     /* superblocks moved from dirty to clean shards in a kernel */

@@ -60,7 +60,7 @@ typedef struct kma_run_stats {
 	uint64_t double_frees;  /* heap counter (native layout) */
 	uint64_t bad_frees;     /* heap counter (native layout) */
 	uint64_t pool_empty;    /* refills that found no free superblock */
-	uint64_t waits[4];      /* watchdog expiries: reserve, claim, wavefront claim, pool */
+	uint64_t waits[4];      /* watchdog expiries: reserve, claim, group claim, pool */
 	uint64_t remote_frees;  /* frees issued from a non-home XCD */
 	uint64_t migrated;      /* superblocks moved from an XCD's dirty shards to
 				   its clean shards inside a kernel */
