@@ -242,7 +242,8 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
  * frees completed jobs' blocks with the whole grid.  A job whose malloc
  * returns NULL waits a step; after a step with such stalls the most recently
  * admitted jobs are preempted (blocks freed, job requeued ahead of new
- * arrivals, restarting from step 0) until their blocks cover the shortfall.
+ * arrivals, restarting from step 0) until their blocks cover the shortfall,
+ * and admission pauses until use falls 10 points below watermark_pct.
  * Job j's result is the sum of the 64-bit patterns its blocks carried in the
  * run that completed it, which depends on (j, its length) only. */
 typedef struct kma_serve_spec {

@@ -24,9 +24,12 @@
  * a step with stalls, phase A preempts the most recently admitted live jobs
  * until their blocks cover the shortfall (their blocks are freed in phase C,
  * without adding to their results) and puts them on a requeue ring that is
- * admitted before new arrivals; no job is admitted after a step with stalls.
- * This keeps the loop from filling the heap with jobs that can then never
- * grow (admission by a use watermark alone cannot know how long jobs grow).
+ * admitted before new arrivals.  A step with stalls also pauses admission
+ * until use falls SV_HYST_PCT points below the watermark (hysteresis), so
+ * the freed room is not refilled at once with jobs that the next steps'
+ * growth would preempt again.  Both keep the loop from filling the heap
+ * with jobs that can then never grow (admission by a use watermark alone
+ * cannot know how long jobs grow).
  * Barriers release (write back L2) and acquire (invalidate) at agent scope,
  * so plain stores of one phase are visible to every XCD in the next.  Each
  * barrier wait has a 5 s watchdog that aborts the whole loop.
@@ -63,6 +66,7 @@ enum : uint32_t {
 	C_RQ_HEAD = 193,
 	C_RQ_TAIL = 194,
 	C_SEQ = 195,
+	C_HOLD = 196,      /* admission paused since a step with stalls */
 	C_POOLED = 224,      /* free superblocks, summed over the work-groups' slices */
 	C_STEP_STALLS = 240, /* stalls of the current step (all work-groups) */
 };
@@ -84,6 +88,7 @@ enum : uint32_t {
 	S_N
 };
 constexpr uint64_t MASK40 = (1ull << 40) - 1ull;
+constexpr uint32_t SV_HYST_PCT = 10; /* admission hysteresis, points below the watermark */
 constexpr uint64_t BARRIER_TICKS = 500000000ull; /* 5 s */
 
 __device__ __forceinline__ uint64_t pattern(uint64_t job, uint64_t k)
@@ -175,6 +180,15 @@ __device__ __forceinline__ void admit(const kma_serve_args &a, uint64_t t0, uint
 	if (used > S[S_PEAK_USED])
 		S[S_PEAK_USED] = used;
 	bool stop = now >= a.duration_ticks || (next >= a.njobs && live == 0 && rqh == rqt);
+	/* hysteresis: a step with stalls pauses admission until use falls
+	 * SV_HYST_PCT points below the watermark, so the heap is not refilled
+	 * with jobs that the next steps' growth would preempt again */
+	bool hold = ctl[C_HOLD] != 0;
+	if (stalls)
+		hold = true;
+	else if (hold && used * 100u < (uint64_t)(a.watermark_pct > SV_HYST_PCT ? a.watermark_pct - SV_HYST_PCT : 0u) * H->nsb)
+		hold = false;
+	ctl[C_HOLD] = hold ? 1u : 0u;
 	if (!stop && stalls) {
 		/* preempt the youngest live jobs until their blocks cover the
 		 * shortfall; their blocks go to this step's free list */
@@ -198,7 +212,7 @@ __device__ __forceinline__ void admit(const kma_serve_args &a, uint64_t t0, uint
 		}
 	}
 	ctl[C_DONE] = dc;
-	if (!stop && !stalls) {
+	if (!stop && !hold) {
 		uint32_t k = 0;
 		while (nfree > 0 && k < a.max_admit && used * 100u < (uint64_t)a.watermark_pct * H->nsb) {
 			uint64_t job;
