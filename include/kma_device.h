@@ -708,7 +708,8 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 		uint32_t sb = (uint32_t)h - 2u;
 		uint32_t g = (uint32_t)(h >> 32);
 		uint64_t old = fadd(state(H, sb), (uint64_t)n);
-		if (st_gen(old) != g) {
+		/* the incarnation read from the head: same generation and class */
+		if (st_gen(old) != g || st_cls1(old) != c + 1u) {
 			undo(H, sb, n, x, ctx);
 			h = fadd(hp, (uint64_t)0);
 			continue;

@@ -7,7 +7,7 @@
  * in host memory, one OS thread per device thread.  Used by the CPU tests to
  * check the allocator's invariants without a GPU.
  *
- * usage: kma_sim sb_log2 sblocks nwg nwi rounds mode seed nctx npool
+ * usage: kma_sim sb_log2 sblocks nwg nwi rounds mode seed nctx npool [variant]
  * prints: ops nulls bad double_frees bad_frees waits canonical_problems
  */
 #include <stdio.h>
@@ -115,6 +115,7 @@ int main(int argc, char **argv)
 	uint64_t seed = strtoull(argv[7], 0, 0);
 	in.nctx = (uint32_t)atoi(argv[8]);
 	in.npool = (uint32_t)atoi(argv[9]);
+	const uint32_t variant = argc > 10 ? (uint32_t)atoi(argv[10]) : 0u;
 	kma_config cfg;
 	nat_hdr nh;
 	char err[200];
@@ -151,7 +152,7 @@ int main(int argc, char **argv)
 				g_sim_lane.bid = {b, 0, 0};
 				g_sim_lane.wg = wgs[b].get();
 				g_sim_lane.wave = waves[b * (KMA_BATCH_BLOCK / 64) + t / 64].get();
-				kma_wl_args wa = {0, rounds, mode, 64, seed, nwi, 0, stats, nullptr};
+				kma_wl_args wa = {0, rounds, mode, 64, seed, nwi, variant, stats, nullptr};
 				wl_batched<true>(H, *lds[b], wa);
 			});
 	for (auto &t : th)

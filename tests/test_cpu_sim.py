@@ -33,12 +33,14 @@ def _run(sim, *args):
 
 
 @pytest.mark.parametrize("args", [
-    # sb_log2 sblocks nwg nwi rounds mode seed nctx npool
+    # sb_log2 sblocks nwg nwi rounds mode seed nctx npool [variant]
     (14, 1024, 4, 1024, 3, 1, 1, 64, 16),     # one context per work-group
     (14, 65536, 16, 4096, 4, 1, 1, 8, 16),    # two work-groups per context
     (12, 4096, 4, 1000, 20, 0, 1, 64, 64),    # fixed 64 B, partial last work-group
     (12, 4096, 4, 777, 10, 1, 5, 64, 64),     # 4 KiB superblocks, mixed classes
     (14, 64, 8, 2048, 3, 1, 9, 8, 16),        # tiny heap: pool exhaustion, NULLs allowed
+    (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 2), # collective malloc, per-lane kma_free
+    (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 1), # per-lane kma_malloc, collective free
 ])
 def test_batched_invariants_in_simulation(sim, args):
     vals, waits, raw = _run(sim, *args)
