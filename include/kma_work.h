@@ -292,6 +292,55 @@ int64_t kma_serve_trace(const kma_serve_spec *spec, uint64_t *arrival, uint32_t 
 /* Lengths of the listed jobs (host only). */
 int kma_serve_lengths(const kma_serve_spec *spec, const uint64_t *jobs, uint64_t n, uint32_t *lengths);
 
+/* ---- graph building: the reference's tb_clTree application ----
+ * Every edge {source, sink} makes sure both node ids are in an insert-only
+ * tree (tb_clTree.cl:12-38 clGraph_node_ensure), allocates a link and
+ * appends it to the source node's adjacency list (tb_clTree.cl:57-67).
+ * Nodes (32 B) and links (16 B) come from
+ *   KMA_GRAPH_PLAIN       one kma_malloc per lane; a node that lost its
+ *                         insert race is freed (clTree_test,
+ *                         tb_clTree.cl:41-85);
+ *   KMA_GRAPH_CACHE       the same, the lost node kept for the lane's next
+ *                         insert (clTree_test_cache, tb_clTree.cl:87-173);
+ *                         also runs on the bump allocator (pma != NULL),
+ *                         like the reference's "poormans heap" run;
+ *   KMA_GRAPH_COLLECTIVE  work-group collective kma_wg_malloc calls for the
+ *                         missing source nodes, the missing sink nodes and
+ *                         the links (clTree_test_al, tb_clTree.cl:175-273).
+ */
+#define KMA_GRAPH_PLAIN 0
+#define KMA_GRAPH_CACHE 1
+#define KMA_GRAPH_COLLECTIVE 2
+
+typedef struct kma_graph_stats {
+	double gpu_ms;         /* the build launch */
+	uint64_t edges;        /* edges linked */
+	uint64_t nodes;        /* distinct node ids in the edge list */
+	uint64_t node_mallocs; /* node allocations, those that lost an insert race included */
+	uint64_t node_frees;   /* nodes freed after a lost race or left in a lane's cache */
+	uint64_t link_mallocs;
+	uint64_t nulls;        /* allocations that returned NULL (that lane stops building) */
+	uint64_t bad;          /* lost insert races whose winner was not found */
+	uint64_t violations;   /* ids missing from the tree, or whose links differ from the edge list */
+	uint64_t links_found;  /* links reached from the tree */
+	uint32_t nwg;          /* work-groups of 256 used */
+	uint32_t pad;
+} kma_graph_stats;
+
+/* An edge list of nedges {source, sink} pairs over nnodes distinct ids
+ * (pseudo-random, from seed), in edges[2 * nedges].  The reference reads
+ * the same pairs from a text file (tb_clTree.c:350-389); none ships. */
+int kma_graph_generate(uint32_t nnodes, uint64_t nedges, uint64_t seed, uint32_t *edges);
+
+/* Build the graph of `edges` (n pairs) with one launch of nwg work-groups of
+ * 256 (0: one per 256 edges, at most 1,024) on the native heap `heap`, or on
+ * the bump allocator `pma` (KMA_GRAPH_CACHE only); then check it against the
+ * edge list on the device, and with `teardown` free every node and link
+ * (native heap only).  Fills *out; 0 on success. */
+struct kma_pma;
+int kma_graph_build(kma_heap *heap, struct kma_pma *pma, const uint32_t *edges, uint64_t n, int mode,
+		    uint32_t nwg, int teardown, kma_graph_stats *out);
+
 /* ------------------------------------------------------------------ */
 /* Deterministic batch mode (reference layout)                          */
 /* ------------------------------------------------------------------ */

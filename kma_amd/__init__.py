@@ -36,7 +36,9 @@ EXPORTS = [
     "kma_queue_items", "kma_queue_run", "kma_list_create", "kma_list_destroy", "kma_list_device",
     "kma_list_run_grow", "kma_list_run_get", "kma_list_clear", "kma_list_pages", "kma_tree_run",
     "kma_batch_malloc_det", "kma_batch_free_det", "kma_serve_run", "kma_serve_trace", "kma_serve_lengths",
+    "kma_graph_generate", "kma_graph_build",
 ]
+GRAPH_PLAIN, GRAPH_CACHE, GRAPH_COLLECTIVE = 0, 1, 2
 TABLE_POW2, TABLE_HALF = 0, 1
 TUNE_MAX = 64
 
@@ -173,6 +175,16 @@ class ServeSpec(ctypes.Structure):
     ]
 
 
+class GraphStats(ctypes.Structure):
+    _fields_ = [("gpu_ms", ctypes.c_double)] + [
+        (k, ctypes.c_uint64) for k in ("edges", "nodes", "node_mallocs", "node_frees", "link_mallocs", "nulls",
+                                       "bad", "violations", "links_found")] + [
+        ("nwg", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("run_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double)] + [
         (k, ctypes.c_uint64) for k in ("jobs", "truncated", "steps", "admitted", "done", "drained", "mallocs",
@@ -279,6 +291,8 @@ def lib():
         L.kma_serve_trace.argtypes = [P(ServeSpec), vp, vp, u64]
         L.kma_serve_trace.restype = i64
         L.kma_serve_lengths.argtypes = [P(ServeSpec), vp, u64, vp]
+        L.kma_graph_generate.argtypes = [u32, u64, u64, vp]
+        L.kma_graph_build.argtypes = [vp, vp, vp, u64, ctypes.c_int, u32, ctypes.c_int, P(GraphStats)]
         L.kma_stream_autotune.argtypes = [vp, u64, u32, u64, u32, P(TuneResult)]
         _lib = L
     return _lib
@@ -684,6 +698,47 @@ def serve_lengths(spec, jobs):
     _check(lib().kma_serve_lengths(ctypes.byref(spec), jobs.ctypes.data, len(jobs), out.ctypes.data),
            "kma_serve_lengths")
     return out[:len(jobs)]
+
+
+def graph_generate(nnodes, nedges, seed):
+    """An edge list: nedges (source, sink) rows over nnodes distinct ids."""
+    out = np.zeros((max(nedges, 1), 2), dtype=np.uint32)
+    _check(lib().kma_graph_generate(nnodes, nedges, seed, out.ctypes.data), "kma_graph_generate")
+    return out[:nedges]
+
+
+def graph_build(edges, mode, heap=None, pma=None, nwg=0, teardown=False):
+    """Build the graph of `edges` ((n, 2) uint32) on a native heap or a bump
+    allocator (kma_graph_build); returns its statistics."""
+    edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+    st = GraphStats()
+    _check(lib().kma_graph_build(heap._h if heap is not None else None, pma._p if pma is not None else None,
+                                 edges.ctypes.data, len(edges), mode, nwg, 1 if teardown else 0,
+                                 ctypes.byref(st)), "kma_graph_build")
+    return st.as_dict()
+
+
+def graph_write(path, edges):
+    """Write an edge list in the reference's data-set format: the number of
+    entries, then one `{source,sink}` line per edge (tb_clTree.c:350-389)."""
+    edges = np.asarray(edges, dtype=np.uint32).reshape(-1, 2)
+    with open(path, "w") as f:
+        f.write("%d\n" % len(edges))
+        f.writelines("{%d,%d}\n" % (int(a), int(b)) for a, b in edges)
+
+
+def graph_read(path):
+    """Read an edge list in the reference's data-set format."""
+    with open(path) as f:
+        n = int(f.readline())
+        rows = []
+        for _ in range(n):
+            line = f.readline().strip()
+            if not (line.startswith("{") and line.endswith("}")):
+                raise ValueError("malformed data-set line: %r" % line)
+            a, b = line[1:-1].split(",")
+            rows.append((int(a), int(b)))
+    return np.array(rows, dtype=np.uint32).reshape(-1, 2)
 
 
 def serve_pattern(job, k):

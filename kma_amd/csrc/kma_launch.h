@@ -66,6 +66,11 @@ int kmad_queue_run(void *q, uint32_t mode, char *d_items, uint64_t n, uint32_t r
 int kmad_list_grow(void *l, const uint32_t *d_objs, uint64_t n, uint32_t block, int local, uint64_t *d_out);
 int kmad_list_get(void *l, const uint64_t *d_idx, uint64_t n, uint64_t *d_out);
 int kmad_list_clear(void *l, uint64_t n);
+int kmad_graph_build(void *heap, void *pma, void *tree, const uint32_t *d_edges, uint64_t n, int mode,
+		     uint32_t nwg, unsigned long long *d_stats, void *stream);
+int kmad_graph_check(void *tree, const uint32_t *d_keys, const uint32_t *d_deg, const uint64_t *d_ssum,
+		     uint64_t nk, uint64_t *d_nodes, unsigned long long *d_stats, void *stream);
+int kmad_graph_free(void *heap, const uint64_t *d_nodes, const uint32_t *d_deg, uint64_t nk, void *stream);
 int kmad_tree_run(void *heap, void *tree, const uint32_t *d_keys, uint64_t n, uint32_t *d_added,
 		  const uint32_t *d_query, uint64_t nq, uint32_t *d_found, void *d_stack,
 		  uint32_t *d_inorder, uint64_t *d_res);
