@@ -178,6 +178,33 @@ SERVE = dict(rate_per_ms=2000.0, len_mu=6.0, len_sigma=1.0, duration_ms=10000.0,
 SERVE_HEAP_SB = 65536  # x 1 MiB = 64 GiB
 
 
+GRAPH = dict(nnodes=200000, nedges=2000000, seed=21)
+
+
+def bench_graph():
+    """Graph building (the reference's tree/queue application) on a 1 GiB
+    native heap: every node and link form, plus the cache form on the bump
+    allocator; edges linked per second of the build launch."""
+    import kma_amd
+
+    edges = kma_amd.graph_generate(GRAPH["nnodes"], GRAPH["nedges"], GRAPH["seed"])
+    out = {"workload": "%d edges over %d node ids (seed %d), 1 GiB heap of 16 KiB superblocks, 1,024 "
+                       "work-groups of 256" % (GRAPH["nedges"], GRAPH["nnodes"], GRAPH["seed"])}
+    for mode, name in ((kma_amd.GRAPH_PLAIN, "plain"), (kma_amd.GRAPH_CACHE, "cache"),
+                       (kma_amd.GRAPH_COLLECTIVE, "collective")):
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 65536) as h:
+            st = kma_amd.graph_build(edges, mode, heap=h, nwg=1024)
+        out[name] = {"edges_per_s": st["edges"] / (st["gpu_ms"] * 1e-3), "ms": st["gpu_ms"],
+                     "violations": int(st["violations"] + st["bad"] + st["nulls"])}
+        log("bench: graph %s %s" % (name, out[name]))
+    with kma_amd.Pma(1 << 30, 16) as p:
+        st = kma_amd.graph_build(edges, kma_amd.GRAPH_CACHE, pma=p, nwg=1024)
+    out["cache_bump_allocator"] = {"edges_per_s": st["edges"] / (st["gpu_ms"] * 1e-3), "ms": st["gpu_ms"],
+                                   "violations": int(st["violations"] + st["bad"] + st["nulls"])}
+    log("bench: graph cache on the bump allocator %s" % out["cache_bump_allocator"])
+    return out
+
+
 def bench_serve(duration_ms):
     import kma_amd
 
@@ -214,6 +241,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--only", choices=["plain", "fused", "batched"],
                     help="run only this form of the full-occupancy workload, then the copy "
                          "probe (the command the counter profiles are taken with)")
@@ -326,6 +354,8 @@ def main():
         out["batched_stream"] = bench_stream(5)
     if not args.no_serve:
         out["serving"] = bench_serve(args.serve_ms)
+    if not args.no_graph:
+        out["graph_build"] = bench_graph()
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline()
