@@ -342,6 +342,48 @@ int kma_graph_build(kma_heap *heap, struct kma_pma *pma, const uint32_t *edges, 
 		    uint32_t nwg, int teardown, kma_graph_stats *out);
 
 /* ------------------------------------------------------------------ */
+/* Heap walker / orphan detector (native layout)                        */
+/* ------------------------------------------------------------------ */
+/* The reference has a commented-out heap walker (kma.cl:528-548) and
+ * prints heap dumps from its test bench (tb_kma.c:304-313); this is the
+ * walker as a checker.  Every superblock must be in exactly one place:
+ *   - free (class 0, count 0, no bit set): in exactly one pool shard chain
+ *     or one context spare slot;
+ *   - in use: its count equals its set bits, no bit past the class's
+ *     slots, its owned bit set iff exactly one context head names it (with
+ *     its generation and class), and not in any free list;
+ *   - an orphan is a superblock with a class, no live block and no owner:
+ *     nobody would ever return it to the pool.
+ * Shard counts must equal the chain lengths, a non-empty shard's mask bit
+ * must be set, the page bitmap must be a sequence of runs whose first pages
+ * record their lengths, and the window summaries of both levels must be
+ * exact.  Only valid while no kernel uses the heap. */
+typedef struct kma_heap_report {
+	uint64_t problems;     /* violations found (0: consistent) */
+	uint64_t sb_total;     /* superblocks */
+	uint64_t sb_free;      /* free superblocks (pool shards + spare slots) */
+	uint64_t sb_in_spares; /* of which in context spare slots */
+	uint64_t sb_heads;     /* superblocks attached to a context head */
+	uint64_t sb_detached;  /* in use, live blocks, no owner (freed back later) */
+	uint64_t orphans;      /* class set, no live block, no owner */
+	uint64_t live_blocks;  /* bitmap bits set */
+	uint64_t live_bytes;   /* live blocks times their class size */
+	uint64_t page_runs;    /* live page runs */
+	uint64_t pages_used;
+	uint64_t pages_total;
+	uint64_t counters[9];  /* the heap's event counters, KMA_STAT_* order */
+	char first[1024];      /* the first problems, one per line */
+} kma_heap_report;
+
+/* Check a metadata image: bytes [0, off_data) of a native heap, as
+ * kma_snapshot(h, 0, ...) returns them.  0 with *out filled (problems
+ * counted there), -1 if the image is not a native heap's. */
+int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out);
+
+/* Snapshot h's metadata and check it (no kernel may be using h). */
+int kma_heap_check(kma_heap *h, kma_heap_report *out);
+
+/* ------------------------------------------------------------------ */
 /* Deterministic batch mode (reference layout)                          */
 /* ------------------------------------------------------------------ */
 /* Serve n mallocs as ONE parallel batch whose result is identical, address

@@ -36,7 +36,7 @@ EXPORTS = [
     "kma_queue_items", "kma_queue_run", "kma_list_create", "kma_list_destroy", "kma_list_device",
     "kma_list_run_grow", "kma_list_run_get", "kma_list_clear", "kma_list_pages", "kma_tree_run",
     "kma_batch_malloc_det", "kma_batch_free_det", "kma_serve_run", "kma_serve_trace", "kma_serve_lengths",
-    "kma_graph_generate", "kma_graph_build",
+    "kma_graph_generate", "kma_graph_build", "kma_image_check", "kma_heap_check",
 ]
 GRAPH_PLAIN, GRAPH_CACHE, GRAPH_COLLECTIVE = 0, 1, 2
 TABLE_POW2, TABLE_HALF = 0, 1
@@ -185,6 +185,19 @@ class GraphStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class HeapReport(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "problems", "sb_total", "sb_free", "sb_in_spares", "sb_heads", "sb_detached", "orphans", "live_blocks",
+        "live_bytes", "page_runs", "pages_used", "pages_total")] + [
+        ("counters", ctypes.c_uint64 * 9), ("first", ctypes.c_char * 1024)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_[:12]}
+        d["counters"] = list(self.counters)
+        d["first"] = self.first.decode(errors="replace").splitlines()
+        return d
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("run_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double)] + [
         (k, ctypes.c_uint64) for k in ("jobs", "truncated", "steps", "admitted", "done", "drained", "mallocs",
@@ -293,6 +306,8 @@ def lib():
         L.kma_serve_lengths.argtypes = [P(ServeSpec), vp, u64, vp]
         L.kma_graph_generate.argtypes = [u32, u64, u64, vp]
         L.kma_graph_build.argtypes = [vp, vp, vp, u64, ctypes.c_int, u32, ctypes.c_int, P(GraphStats)]
+        L.kma_image_check.argtypes = [vp, u64, P(HeapReport)]
+        L.kma_heap_check.argtypes = [vp, P(HeapReport)]
         L.kma_stream_autotune.argtypes = [vp, u64, u32, u64, u32, P(TuneResult)]
         _lib = L
     return _lib
@@ -385,6 +400,14 @@ class Heap:
 
     def reset(self):
         _check(lib().kma_reset(self._h), "kma_reset")
+
+    def check(self):
+        """Walk a quiescent native heap (kma_heap_check): a dict whose
+        `problems` is 0 when every superblock, shard and page run is
+        consistent; `first` lists the first problems found."""
+        r = HeapReport()
+        _check(lib().kma_heap_check(self._h, ctypes.byref(r)), "kma_heap_check")
+        return r.as_dict()
 
     def trim(self):
         _check(lib().kma_trim(self._h), "kma_trim")
@@ -698,6 +721,15 @@ def serve_lengths(spec, jobs):
     _check(lib().kma_serve_lengths(ctypes.byref(spec), jobs.ctypes.data, len(jobs), out.ctypes.data),
            "kma_serve_lengths")
     return out[:len(jobs)]
+
+
+def image_check(image):
+    """kma_image_check on a metadata image (a uint8 array holding bytes
+    [0, off_data) of a native heap)."""
+    image = np.ascontiguousarray(image, dtype=np.uint8)
+    r = HeapReport()
+    _check(lib().kma_image_check(image.ctypes.data, image.size, ctypes.byref(r)), "kma_image_check")
+    return r.as_dict()
 
 
 def graph_generate(nnodes, nedges, seed):

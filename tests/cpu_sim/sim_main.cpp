@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <memory>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -97,10 +98,25 @@ static int canonical_problems(kma_dheap *H)
 	return probs;
 }
 
+static bool dump_image(const char *path, const char *suffix, const char *buf, uint64_t n)
+{
+	std::string name = std::string(path) + suffix;
+	FILE *f = fopen(name.c_str(), "wb");
+	bool ok = f && fwrite(buf, 1, n, f) == n;
+	if (f)
+		ok = fclose(f) == 0 && ok;
+	if (!ok)
+		fprintf(stderr, "cannot write %s\n", name.c_str());
+	return ok;
+}
+
 int main(int argc, char **argv)
 {
 	if (argc < 10) {
-		fprintf(stderr, "usage: %s sb_log2 sblocks nwg nwi rounds mode seed nctx npool\n", argv[0]);
+		fprintf(stderr, "usage: %s sb_log2 sblocks nwg nwi rounds mode seed nctx npool [variant [npages]]\n"
+				"  KMA_SIM_DUMP=path writes the metadata image after the run to path.run\n"
+				"  and after the trim to path.trim\n",
+			argv[0]);
 		return 2;
 	}
 	kma_config in;
@@ -116,6 +132,8 @@ int main(int argc, char **argv)
 	in.nctx = (uint32_t)atoi(argv[8]);
 	in.npool = (uint32_t)atoi(argv[9]);
 	const uint32_t variant = argc > 10 ? (uint32_t)atoi(argv[10]) : 0u;
+	in.npages = argc > 11 ? strtoull(argv[11], 0, 0) : 0u;
+	const char *dump = getenv("KMA_SIM_DUMP");
 	kma_config cfg;
 	nat_hdr nh;
 	char err[200];
@@ -158,7 +176,11 @@ int main(int argc, char **argv)
 	for (auto &t : th)
 		t.join();
 	uint64_t *st = kma_nat::stats(H);
+	if (dump && !dump_image(dump, ".run", buf, nh.off_data))
+		return 3;
 	trim(H);
+	if (dump && !dump_image(dump, ".trim", buf, nh.off_data))
+		return 3;
 	int probs = canonical_problems(H);
 	printf("ops %llu nulls %llu bad %llu double_frees %llu bad_frees %llu waits %llu %llu %llu %llu canonical_problems %d\n",
 	       stats[0], stats[1], stats[2], (unsigned long long)st[0], (unsigned long long)st[1],
