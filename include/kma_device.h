@@ -734,6 +734,30 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 	}
 }
 
+/* Bits [lo, hi) that fall in bitmap word wi. */
+__device__ __forceinline__ uint32_t range_bits(uint32_t wi, uint32_t lo, uint32_t hi)
+{
+	uint32_t a = max(lo, wi * 32u), b = min(hi, wi * 32u + 32u);
+	if (a >= b)
+		return 0u;
+	uint32_t n = b - a;
+	return (n == 32u ? ~0u : ((1u << n) - 1u)) << (a - wi * 32u);
+}
+
+/* Lane of the k-th (from 0) set bit of m; m has more than k bits set. */
+__device__ __forceinline__ uint32_t nth_lane(uint64_t m, uint32_t k)
+{
+	uint32_t pos = 0;
+	for (uint32_t s = 32; s; s >>= 1) {
+		uint32_t n = (uint32_t)__popcll((m >> pos) & ((1ull << s) - 1ull));
+		if (n <= k) {
+			k -= n;
+			pos += s;
+		}
+	}
+	return pos;
+}
+
 /* Claim one free bit at or after `start` (wrapping) in superblock sb.  The
  * reserved slot `start` itself is tried first with no read (blocks are
  * claimed bottom-up and mostly freed in bulk, so it is usually free); then
@@ -1153,7 +1177,12 @@ __device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
 	return (int64_t)s < 0 ? 0u : s;
 }
 
-/* Allocate `size` bytes; NULL for size 0 or when the heap cannot serve it. */
+/* Allocate `size` bytes; NULL for size 0 or when the heap cannot serve it.
+ * The calling lanes of a wavefront are served together: the first lane of
+ * each size class reserves slots for its class's lanes (every class at
+ * once), then the first lane in each bitmap word of a class's reserved
+ * slots sets that word's bits with one RMW for all of them; a lane whose
+ * slot was already taken scans for another free bit. */
 __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 {
 	using namespace kma_nat;
@@ -1169,29 +1198,39 @@ __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 	char *result = nullptr;
 	bool pending = true;
 	for (;;) {
-		uint64_t act = __ballot(pending);
-		if (act == 0)
+		uint64_t left = __ballot(pending);
+		if (left == 0)
 			break;
-		uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
-		uint32_t lc = rdlane(c, leader);
-		bool member = pending && c == lc;
-		uint64_t grp = __ballot(member);
-		uint32_t n = (uint32_t)__popcll(grp);
+		/* class groups of the pending lanes (no memory traffic) */
+		uint64_t grp = 0;
+		do {
+			uint32_t l0 = (uint32_t)__ffsll((unsigned long long)left) - 1u;
+			uint32_t lc = rdlane(c, l0);
+			uint64_t g = __ballot(pending && c == lc);
+			if (pending && c == lc)
+				grp = g;
+			left &= ~g;
+		} while (left);
+		const uint32_t leader = pending ? (uint32_t)__ffsll((unsigned long long)grp) - 1u : lane();
 		resv r = {0, 0, 0, 0};
-		if (lane() == leader)
-			r = reserve(H, x, ctx, lc, n);
-		uint32_t sb = rdlane(r.sb, leader);
-		uint32_t rbase = rdlane(r.base, leader);
-		uint32_t got = rdlane(r.got, leader);
-		uint32_t ok = rdlane(r.ok, leader);
-		if (member) {
-			uint32_t q = (uint32_t)__popcll(grp & lanemask_lt());
-			if (!ok) {
-				pending = false;
-			} else if (q < got) {
-				result = claim(H, sb, c, rbase + q);
-				pending = false;
-			}
+		if (pending && lane() == leader)
+			r = reserve(H, x, ctx, c, (uint32_t)__popcll(grp));
+		const uint32_t sb = __shfl(r.sb, (int)leader), rbase = __shfl(r.base, (int)leader);
+		const uint32_t got = __shfl(r.got, (int)leader), ok = __shfl(r.ok, (int)leader);
+		const uint32_t q = (uint32_t)__popcll(grp & lanemask_lt());
+		const bool mine = pending && ok && q < got;
+		if (pending && !ok)
+			pending = false;
+		const uint32_t slot = rbase + q, wi = slot >> 5;
+		uint32_t old = 0;
+		if (mine && (q == 0u || (slot & 31u) == 0u))
+			old = f_or(bitmap(H, sb) + wi, range_bits(wi, rbase, rbase + got));
+		const uint32_t first = (wi << 5) > rbase ? (wi << 5) - rbase : 0u;
+		old = __shfl(old, (int)(mine ? nth_lane(grp, first) : lane()));
+		if (mine) {
+			const uint32_t m = 1u << (slot & 31u);
+			result = !(old & m) ? sbdata(H, sb) + (uint64_t)slot * H->class_size[c] : claim(H, sb, c, slot);
+			pending = false;
 		}
 	}
 	return result;
@@ -1242,25 +1281,23 @@ __device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 	} else {
 		bump_stat(H, KMA_STAT_BAD_FREE);
 	}
-	/* one count update per superblock per wavefront */
+	/* one count update per superblock per wavefront, every superblock at
+	 * once (groups are formed first, with no memory traffic) */
 	uint32_t ctx = ctx_of(H, x);
-	bool pending = live;
-	for (;;) {
-		uint64_t act = __ballot(pending);
-		if (act == 0)
-			break;
-		uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
-		uint32_t lsb = rdlane(sb, leader);
-		bool member = pending && sb == lsb;
-		uint64_t grp = __ballot(member);
-		if (lane() == leader) {
-			uint32_t n = (uint32_t)__popcll(grp);
-			uint64_t ns = fsub(state(H, lsb), (uint64_t)n) - n;
-			if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
-				try_release(H, lsb, ns, x, ctx);
-		}
-		if (member)
-			pending = false;
+	uint64_t left = __ballot(live), grp = 0;
+	while (left) {
+		uint32_t l0 = (uint32_t)__ffsll((unsigned long long)left) - 1u;
+		uint32_t lsb = rdlane(sb, l0);
+		uint64_t g = __ballot(live && sb == lsb);
+		if (live && sb == lsb)
+			grp = g;
+		left &= ~g;
+	}
+	if (live && lane() == (uint32_t)__ffsll((unsigned long long)grp) - 1u) {
+		uint32_t n = (uint32_t)__popcll(grp);
+		uint64_t ns = fsub(state(H, sb), (uint64_t)n) - n;
+		if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
+			try_release(H, sb, ns, x, ctx);
 	}
 }
 

@@ -72,16 +72,6 @@ __device__ __forceinline__ uint32_t group_bcast(uint32_t v)
 	return __shfl(v, 0, (int)KMA_CLAIM_G);
 }
 
-/* Bits [lo, hi) that fall in bitmap word wi. */
-__device__ __forceinline__ uint32_t range_bits(uint32_t wi, uint32_t lo, uint32_t hi)
-{
-	uint32_t a = max(lo, wi * 32u), b = min(hi, wi * 32u + 32u);
-	if (a >= b)
-		return 0u;
-	uint32_t n = b - a;
-	return (n == 32u ? ~0u : ((1u << n) - 1u)) << (a - wi * 32u);
-}
-
 /* Each group with want set claims `got` free bits of its superblock sb
  * (class c) and writes block offsets (from the heap base) into
  * res[order[0..done-1]]; returns the group's `done`.  The first pass claims
