@@ -102,7 +102,7 @@ typedef struct kma_dheap {
 #define KMA_STAT_DOUBLE_FREE 0
 #define KMA_STAT_BAD_FREE 1
 #define KMA_STAT_POOL_EMPTY 2
-#define KMA_STAT_WAIT_RESERVE 3 /* a reserve() wait hit the watchdog */
+#define KMA_STAT_WAIT_RESERVE 3 /* a reservation kept finding its head busy (or retrying) for the watchdog period */
 #define KMA_STAT_WAIT_CLAIM 4   /* a per-lane bitmap claim hit the watchdog */
 #define KMA_STAT_WAIT_WGCLAIM 5 /* a group claim hit the watchdog, or a whole-superblock claim lost bits */
 #define KMA_STAT_WAIT_POOL 6    /* a pool push/pop hit the watchdog */
@@ -660,13 +660,18 @@ struct resv {
 };
 
 /* Reserve up to n slots of class c from context ctx's current superblock
- * (one lane executes this for its wavefront or work-group group). */
+ * (one lane executes this for its wavefront or work-group group).  A head
+ * that another lane is refilling (HEAD_BUSY) is not waited for: the call
+ * returns {ok 1, got 0} and the caller's wavefront-uniform loop retries.
+ * A lane spinning here on a lock held by a lane of its own wavefront could
+ * keep that lane masked off until the watchdog, and with every class of a
+ * wavefront served at once, a wavefront holding one head while a lane of
+ * it waits on another can close a cycle with a second wavefront. */
 __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c, uint32_t n)
 {
 	uint64_t *hp = head(H, ctx, c);
 	uint32_t cap = H->class_cap[c];
 	waiter wt;
-	uint32_t bo = 1;
 	uint64_t h = ld(hp);
 	for (;;) {
 		if (wt.expired()) {
@@ -699,11 +704,8 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 			return resv{sb, cnt, got, 1};
 		}
 		if (h == HEAD_BUSY) {
-			for (uint32_t i = 0; i < bo; i++)
-				__builtin_amdgcn_s_sleep(2);
-			bo = bo < 4u ? 2u * bo : 4u;
-			h = fadd(hp, (uint64_t)0);
-			continue;
+			__builtin_amdgcn_s_sleep(2);
+			return resv{0, 0, 0, 1};
 		}
 		uint32_t sb = (uint32_t)h - 2u;
 		uint32_t g = (uint32_t)(h >> 32);
@@ -1178,11 +1180,13 @@ __device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
 }
 
 /* Allocate `size` bytes; NULL for size 0 or when the heap cannot serve it.
- * The calling lanes of a wavefront are served together: the first lane of
- * each size class reserves slots for its class's lanes (every class at
- * once), then the first lane in each bitmap word of a class's reserved
- * slots sets that word's bits with one RMW for all of them; a lane whose
- * slot was already taken scans for another free bit. */
+ * The calling lanes of a wavefront are served together: grouped by size
+ * class with ballots, the first lane of each class reserves slots for its
+ * class from the context's head (every class at once), then the first lane
+ * in each bitmap word of a class's reserved slots sets that word's bits with
+ * one RMW for all of them; a lane whose slot was already taken scans for
+ * another free bit.  Lanes left without a slot (a busy head, a superblock
+ * that filled up) go round again. */
 __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 {
 	using namespace kma_nat;
@@ -1192,15 +1196,22 @@ __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 		uint32_t seed = (blockIdx.x * blockDim.x + threadIdx.x) ^ (uint32_t)clock64();
 		return page_alloc(H, size, seed);
 	}
-	uint32_t c = (uint32_t)class_of(H, (uint32_t)size);
-	uint32_t x = xcc();
-	uint32_t ctx = ctx_of(H, x);
+	const uint32_t c = (uint32_t)class_of(H, (uint32_t)size);
+	const uint32_t x = xcc();
+	const uint32_t ctx = ctx_of(H, x);
 	char *result = nullptr;
 	bool pending = true;
+	waiter wt;
 	for (;;) {
 		uint64_t left = __ballot(pending);
 		if (left == 0)
 			break;
+		if (__ballot(wt.expired())) {
+			/* a head stayed busy for the whole watchdog period */
+			if (lane() == (uint32_t)__ffsll((unsigned long long)left) - 1u)
+				fadd(stats(H) + KMA_STAT_WAIT_RESERVE, (uint64_t)__popcll(left));
+			break;
+		}
 		/* class groups of the pending lanes (no memory traffic) */
 		uint64_t grp = 0;
 		do {
@@ -1211,13 +1222,13 @@ __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 				grp = g;
 			left &= ~g;
 		} while (left);
-		const uint32_t leader = pending ? (uint32_t)__ffsll((unsigned long long)grp) - 1u : lane();
-		resv r = {0, 0, 0, 0};
-		if (pending && lane() == leader)
-			r = reserve(H, x, ctx, c, (uint32_t)__popcll(grp));
-		const uint32_t sb = __shfl(r.sb, (int)leader), rbase = __shfl(r.base, (int)leader);
-		const uint32_t got = __shfl(r.got, (int)leader), ok = __shfl(r.ok, (int)leader);
 		const uint32_t q = (uint32_t)__popcll(grp & lanemask_lt());
+		resv r = {0, 0, 0, 0};
+		if (pending && q == 0u)
+			r = reserve(H, x, ctx, c, (uint32_t)__popcll(grp));
+		const int src = (int)(pending ? (uint32_t)__ffsll((unsigned long long)grp) - 1u : lane());
+		const uint32_t sb = __shfl(r.sb, src), rbase = __shfl(r.base, src);
+		const uint32_t got = __shfl(r.got, src), ok = __shfl(r.ok, src);
 		const bool mine = pending && ok && q < got;
 		if (pending && !ok)
 			pending = false;

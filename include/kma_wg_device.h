@@ -251,7 +251,14 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 		if (__ballot(nwhole != 0u))
 			r0 = whole_superblocks(H, x, ctx, k, cap, gl < nwhole, ord, L.res);
 		bool pend = r0 < n;
+		waiter wt;
 		while (__ballot(pend)) {
+			if (__ballot(wt.expired())) {
+				/* a head stayed busy for the whole watchdog period */
+				if (pend && gl == 0)
+					bump_stat(H, KMA_STAT_WAIT_RESERVE);
+				break;
+			}
 			resv r = {0, 0, 0, 0};
 			if (pend && gl == 0)
 				r = reserve(H, x, ctx, k, n - r0);
