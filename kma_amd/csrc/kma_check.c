@@ -96,6 +96,31 @@ int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 		set_err("not a native heap image (or truncated before its data offset)");
 		return -1;
 	}
+	/* every table must lie inside the metadata (a corrupt header must not
+	 * send the walk outside the image) */
+	const uint64_t nw = (h.npages + 63) >> 6, ns_ = (nw + 63) >> 6, ns2_ = (ns_ + 63) >> 6;
+	const struct {
+		uint64_t off, len;
+	} tab[] = {
+		{h.off_state, h.nsb * 8},
+		{h.off_next, h.nsb * 4},
+		{h.off_bitmap, h.nsb * h.bm_words * 4},
+		{h.off_heads, (uint64_t)h.nctx * NAT_HEADS_PER_CTX * 8},
+		{h.off_pool, (uint64_t)h.npool * NAT_POOL_STRIDE_BYTES},
+		{h.off_pbitmap, nw * 8},
+		{h.off_plen, h.npages},
+		{h.off_stats, NAT_STATS_BYTES},
+		{h.off_psum, (uint64_t)NAT_PAGE_LEVELS * (ns_ + ns2_) * 8},
+		{h.off_spare, (uint64_t)h.nctx * 128},
+	};
+	int bad = h.nsb > 0xfffffff0ull || h.npages > (1ull << 40) || h.bm_words > 4096 || h.npool > 512 ||
+		  h.nctx > 65536;
+	for (unsigned i = 0; i < sizeof(tab) / sizeof(tab[0]) && !bad; i++)
+		bad = tab[i].off > h.off_data || tab[i].len > h.off_data - tab[i].off;
+	if (bad) {
+		set_err("native heap image with tables outside its metadata");
+		return -1;
+	}
 	rep p = {out, 0};
 	const uint64_t nsb = h.nsb, W = h.bm_words, S = h.npool / 8;
 	uint8_t *where = (uint8_t *)calloc(nsb + 1, 1); /* free places holding each superblock */

@@ -256,3 +256,14 @@ def test_not_an_image():
 def test_truncated_image(images):
     with pytest.raises(kma_amd.KmaError):
         kma_amd.image_check(images["trim"][:1000])
+
+
+def test_corrupt_table_offsets(images):
+    """A header whose tables point past the metadata is refused, not walked."""
+    for key, value in (("off_bitmap", 1 << 40), ("nsb", 1 << 33), ("off_spare", None)):
+        m = Img(images["trim"])
+        h = np.frombuffer(m.b[: natimg.HDR_DTYPE.itemsize].tobytes(), dtype=natimg.HDR_DTYPE).copy()
+        h[0][key] = value if value is not None else int(h[0]["off_data"]) - 8
+        m.b[: natimg.HDR_DTYPE.itemsize] = np.frombuffer(h.tobytes(), dtype=np.uint8)
+        with pytest.raises(kma_amd.KmaError):
+            kma_amd.image_check(m.b)
