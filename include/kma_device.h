@@ -760,51 +760,70 @@ __device__ __forceinline__ uint32_t nth_lane(uint64_t m, uint32_t k)
 	return pos;
 }
 
-/* Claim one free bit at or after `start` (wrapping) in superblock sb.  The
- * reserved slot `start` itself is tried first with no read (blocks are
- * claimed bottom-up and mostly freed in bulk, so it is usually free); then
- * the scan continues from the word the RMW returned.  Words are read with
- * plain atomic loads; after a wrap they are re-read with RMWs, so a stale
- * cached word cannot hide free bits for long. */
-__device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t start)
+/* Lowest-but-k free bit: the position of the k-th (from 0) set bit of w;
+ * w has more than k bits set. */
+__device__ __forceinline__ uint32_t nth_bit(uint32_t w, uint32_t k)
+{
+	uint32_t pos = 0;
+	for (uint32_t s = 16; s; s >>= 1) {
+		uint32_t n = (uint32_t)__popc((w >> pos) & ((1u << s) - 1u));
+		if (n <= k) {
+			k -= n;
+			pos += s;
+		}
+	}
+	return pos;
+}
+
+/* Scan superblock sb (class c) for a free bit, from word `wi` whose value
+ * `w` the caller has just read.  In each word the lane takes the
+ * (skip mod n)-th of its n free bits, so lanes that scan together (skip =
+ * their rank) spread over different bits instead of all trying the lowest.
+ * Words after the first are read with plain atomic loads; after a wrap
+ * they are re-read with RMWs, so a stale cached word cannot hide free bits
+ * for long.  NULL (counted) when the watchdog expires. */
+__device__ inline char *claim_scan(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t wi, uint32_t w,
+				   uint32_t skip)
 {
 	uint32_t cap = H->class_cap[c];
 	uint32_t *bm = bitmap(H, sb);
 	uint32_t lastw = (cap - 1u) >> 5;
 	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
-	uint32_t b = start < cap ? start : 0u;
-	uint32_t wi = b >> 5;
-	uint32_t m = 1u << (b & 31u);
-	uint32_t w = f_or(bm + wi, m);
-	if (!(w & m))
-		return sbdata(H, sb) + (uint64_t)b * H->class_size[c];
-	w |= m;
 	bool fresh = false;
 	waiter wt;
 	while (!wt.expired()) {
-		uint32_t avail = ~w & (~0u << (b & 31u));
-		if (wi == lastw)
-			avail &= tail;
+		uint32_t avail = ~w & (wi == lastw ? tail : ~0u);
 		if (avail) {
-			uint32_t bit = __ffs(avail) - 1u;
-			m = 1u << bit;
+			uint32_t bit = nth_bit(avail, skip % (uint32_t)__popc(avail));
+			uint32_t m = 1u << bit;
 			uint32_t old = f_or(bm + wi, m);
 			if (!(old & m))
 				return sbdata(H, sb) + (uint64_t)(wi * 32u + bit) * H->class_size[c];
 			w = old | m;
-			b = wi * 32u + bit;
 			continue;
 		}
-		b = (wi + 1u) << 5;
-		if (b >= cap) {
-			b = 0;
+		if (++wi > lastw) {
+			wi = 0;
 			fresh = true;
 		}
-		wi = b >> 5;
 		w = fresh ? f_or(bm + wi, 0u) : ld(bm + wi);
 	}
 	bump_stat(H, KMA_STAT_WAIT_CLAIM);
 	return nullptr;
+}
+
+/* Claim one free bit of superblock sb, trying the reserved slot `start`
+ * itself first with no read (blocks are claimed bottom-up and mostly freed
+ * in bulk, so it is usually free), then scanning from its word. */
+__device__ inline char *claim(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t start, uint32_t skip = 0u)
+{
+	uint32_t cap = H->class_cap[c];
+	uint32_t b = start < cap ? start : 0u;
+	uint32_t m = 1u << (b & 31u);
+	uint32_t w = f_or(bitmap(H, sb) + (b >> 5), m);
+	if (!(w & m))
+		return sbdata(H, sb) + (uint64_t)b * H->class_size[c];
+	return claim_scan(H, sb, c, b >> 5, w | m, skip);
 }
 
 /* ---- large requests: page runs ----
@@ -1238,9 +1257,14 @@ __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 			old = f_or(bitmap(H, sb) + wi, range_bits(wi, rbase, rbase + got));
 		const uint32_t first = (wi << 5) > rbase ? (wi << 5) - rbase : 0u;
 		old = __shfl(old, (int)(mine ? nth_lane(grp, first) : lane()));
+		/* lanes whose slot was taken scan on from the word's new value,
+		 * spread over its free bits by their rank among such lanes */
+		const uint32_t m = 1u << (slot & 31u);
+		const bool lost = mine && (old & m);
+		const uint32_t rank = (uint32_t)__popcll(__ballot(lost) & lanemask_lt());
 		if (mine) {
-			const uint32_t m = 1u << (slot & 31u);
-			result = !(old & m) ? sbdata(H, sb) + (uint64_t)slot * H->class_size[c] : claim(H, sb, c, slot);
+			result = !lost ? sbdata(H, sb) + (uint64_t)slot * H->class_size[c]
+				       : claim_scan(H, sb, c, wi, old | range_bits(wi, rbase, rbase + got), rank);
 			pending = false;
 		}
 	}

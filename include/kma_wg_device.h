@@ -273,7 +273,7 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 				uint32_t mine = 0;
 				if (want)
 					for (uint32_t q = gl; q < got; q += G) {
-						char *p = claim(H, sb, k, rb + q);
+						char *p = claim(H, sb, k, rb + q, gl);
 						L.res[ord[r0 + q]] = p ? (uint64_t)(p - base(H)) : 0;
 						mine += p ? 1u : 0u;
 					}

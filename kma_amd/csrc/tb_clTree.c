@@ -203,6 +203,8 @@ static int run_form(unsigned f, const uint32_t *edges, uint64_t n)
 
 int main(int argc, char **argv)
 {
+	/* one line at a time, so a run that ends abnormally keeps its lines */
+	setvbuf(stdout, NULL, _IOLBF, 0);
 	printf("Tree test using the gfx950 device allocator\n");
 	if (tb_options_read(argc, argv, tree_opts)) {
 		usage();

@@ -224,6 +224,8 @@ static int run_kernel(unsigned k)
 
 int main(int argc, char **argv)
 {
+	/* one line at a time, so a run that ends abnormally keeps its lines */
+	setvbuf(stdout, NULL, _IOLBF, 0);
 	printf("KMA device allocator bench (gfx950)\n");
 	if (tb_options_read(argc, argv, kma_opts)) {
 		usage();
