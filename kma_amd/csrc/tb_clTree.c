@@ -188,7 +188,7 @@ static int run_form(unsigned f, const uint32_t *edges, uint64_t n)
 		}
 		if (rc)
 			break;
-		printf("Threads: %-5" PRIu64, threads);
+		printf("Threads: %-5" PRIu64 " ", threads);
 		tb_print_times(us, samples);
 		if (tb_opt.debug)
 			printf("\tedges %" PRIu64 " nodes %" PRIu64 " node_mallocs %" PRIu64 " node_frees %" PRIu64
