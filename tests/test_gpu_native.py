@@ -125,6 +125,28 @@ def test_full_occupancy_native(form):
         assert natimg.NatHeap(h).canonical_problems() == []
 
 
+@pytest.mark.parametrize("mode", [kma_amd.WL_REF_TEST, kma_amd.WL_LOWVAR, kma_amd.WL_HIGHVAR])
+def test_reference_kernels_full_occupancy_native(mode):
+    """The reference's bench kernels (kma.cl:424-526) at 524,288 work-items
+    on a native heap, one launch of 20 iterations: every lane of a wavefront
+    asks for the same one or two classes, so most of each wavefront's
+    reserved slots are taken by scattered live blocks and its lanes scan for
+    others.  Exact op counts, no NULL, no watchdog expiry, and a time bound
+    that the lanes' scans colliding on the same free bits would break
+    (0.377 s for 100 iterations of kma_test_malloc when they did, 0.026 s
+    without)."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16384) as h:
+        _say("reference kernel mode=%d, full occupancy, native" % mode)
+        st = h.run(mode, 524288, 20, form=kma_amd.FORM_FUSED, block=256)
+        extra = 0 if mode == kma_amd.WL_REF_TEST else 2 * 20
+        assert st["ops"] == 2 * 524288 * 20 + extra, st
+        assert st["nulls"] == 0 and st["bad"] == 0 and st["waits"] == [0, 0, 0, 0], st
+        assert st["double_frees"] == 0 and st["bad_frees"] == 0, st
+        assert st["gpu_ms"] < 40, st
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
 @pytest.mark.parametrize("form", [kma_amd.FORM_PLAIN, kma_amd.FORM_BATCHED])
 def test_cross_xcd_supply_by_migration(form):
     """A heap of 16 superblocks (two per XCD, four 4 KiB blocks each) under
