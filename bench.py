@@ -32,6 +32,11 @@ NWI = 524288
 BLOCK = 256
 SB_LOG2 = 14
 SBLOCKS = 65536  # 1 GiB
+# Allocation contexts of the full-occupancy heap: 4,096 = one per resident
+# wavefront (256 CUs x 4 SIMDs x 4 waves at the kernels' register use), so
+# per-lane calls of different wavefronts do not share a context's heads.
+# The library's default (a quarter of the heap in heads at most) gives 1,024.
+NCTX = 4096
 SEED = 1
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 30  # the copy probe: reads and writes 1 GiB
@@ -260,7 +265,7 @@ def main():
     K, W = args.steps, args.warmup
     if args.only:
         form = {"plain": kma_amd.FORM_PLAIN, "fused": kma_amd.FORM_FUSED, "batched": kma_amd.FORM_BATCHED}[args.only]
-        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS) as h:
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS, nctx=NCTX) as h:
             st = h.run(kma_amd.WL_LOGUNI, NWI, K, form=form, block=BLOCK, seed=SEED, warmup=W)
         gbps, cms = kma_amd.copy_probe(COPY_BYTES, warmup=1, reps=1)
         print(json.dumps({"form": args.only, "rounds": K, "launches": st["launches"],
@@ -270,7 +275,7 @@ def main():
     res = {}
     for form, name in ((kma_amd.FORM_PLAIN, "plain"), (kma_amd.FORM_FUSED, "fused"),
                        (kma_amd.FORM_BATCHED, "batched")):
-        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS) as h:
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS, nctx=NCTX) as h:
             log("bench: %s form, %d warmup + %d timed rounds" % (name, W, K))
             st = h.run(kma_amd.WL_LOGUNI, NWI, K, form=form, block=BLOCK, seed=SEED, warmup=W)
             st["ops_per_s"] = st["ops"] / (st["gpu_ms"] * 1e-3)
@@ -283,7 +288,7 @@ def main():
     # timed runs above carry no probes
     lat, phases = {}, {}
     for form, name in ((kma_amd.FORM_PLAIN, "plain"), (kma_amd.FORM_BATCHED, "batched")):
-        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS) as h:
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS, nctx=NCTX) as h:
             st = h.run(kma_amd.WL_LOGUNI, NWI, 5, form=form, block=BLOCK, seed=SEED, warmup=W,
                        variant=kma_amd.VAR_LATENCY)
             lat[name] = {"p50_ns": st["lat_p50_ns"], "p99_ns": st["lat_p99_ns"], "max_ns": st["lat_max_ns"]}
@@ -321,9 +326,10 @@ def main():
         "dtype": "u32/u64 integer",
         "data": "synthetic (device-generated splitmix64 trace, seed 1)",
         "config": {
-            "workload": "full occupancy: 1 GiB heap, 16 KiB superblocks, 524,288 work-items "
-                        "(2,048 x 256), sizes 2^U{3..11} B, hold U{0..64} u32 writes, seed 1",
+            "workload": "full occupancy: 1 GiB heap, 16 KiB superblocks, 4,096 allocation contexts, "
+                        "524,288 work-items (2,048 x 256), sizes 2^U{3..11} B, hold U{0..64} u32 writes, seed 1",
             "heap_bytes": 1 << 30,
+            "contexts": NCTX,
             "work_items": NWI,
             "rounds_timed": K,
             "form": "batched persistent (one launch, all rounds)",
