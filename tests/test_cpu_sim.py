@@ -41,6 +41,7 @@ def _run(sim, *args):
     (14, 64, 8, 2048, 3, 1, 9, 8, 16),        # tiny heap: pool exhaustion, NULLs allowed
     (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 2), # collective malloc, per-lane kma_free
     (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 1), # per-lane kma_malloc, collective free
+    (14, 1024, 32, 8192, 3, 1, 3, 8, 16, 3),  # per-lane malloc and free, 16 wavefronts per context
 ])
 def test_batched_invariants_in_simulation(sim, args):
     vals, waits, raw = _run(sim, *args)
@@ -53,3 +54,10 @@ def test_batched_invariants_in_simulation(sim, args):
     assert int(vals["ops"]) == 2 * nwi * rounds - nulls, raw
     if args[1] >= 1024:
         assert nulls == 0, raw
+
+
+def test_bit_helpers_brute_force(sim):
+    """nth_lane, nth_bit and range_bits of the device header against plain
+    loops over every bit (tests/cpu_sim/helpers_main.cpp)."""
+    out = subprocess.run([os.path.join(HERE, "_build", "kma_helpers")], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "helpers ok" in out.stdout, out.stdout + out.stderr
