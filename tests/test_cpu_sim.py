@@ -61,3 +61,29 @@ def test_bit_helpers_brute_force(sim):
     loops over every bit (tests/cpu_sim/helpers_main.cpp)."""
     out = subprocess.run([os.path.join(HERE, "_build", "kma_helpers")], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0 and "helpers ok" in out.stdout, out.stdout + out.stderr
+
+
+@pytest.mark.parametrize("args", [
+    # log2 sblocks nwg nwi rounds mode
+    (12, 127, 32, 8192, 30, 4),  # kma_test_malloc_highvar on kma_create(127)
+    (12, 16, 16, 4096, 30, 2),   # kma_test_malloc, 16 superblocks
+    (12, 16, 16, 4096, 30, 3),   # kma_test_malloc_lowvar
+    (12, 8, 32, 8192, 20, 4),    # kma_test_malloc_highvar, 8 superblocks
+])
+def test_reference_layout_in_simulation(sim, args):
+    """The reference-layout device allocator (kma_ref_device.h) run by the
+    reference's bench kernels on CPU threads, built with the host
+    AddressSanitizer (any access outside the heap buffer aborts the run):
+    no read-back mismatch, and after the last free the superblock free list
+    walked by the oracle is back to its initial S - 1 entries."""
+    exe = os.path.join(HERE, "_build", "kma_sim_ref_asan")
+    out = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    f = out.stdout.split()
+    vals = dict(zip(f[0::2], f[1::2]))
+    assert int(vals["bad"]) == 0, out.stdout
+    assert int(vals["queue"]) == args[1] - 1, out.stdout
+    nwi, rounds = args[3], args[4]
+    extra = 0 if args[5] == 2 else 2 * rounds  # work-item 0's 4000-B block
+    if int(vals["nulls"]) == 0:  # a NULL ends its work-item early
+        assert int(vals["ops"]) == 2 * nwi * rounds + extra, out.stdout
