@@ -1,6 +1,6 @@
 /*
  * kma_stream.c - batched request streams: host-side trace generation, the
- * run driver (one launch per batch, or one graph-captured persistent launch)
+ * run loop (one launch per batch, or one graph-captured persistent launch)
  * and the stream autotuner.  C over the HIP runtime C API.
  */
 #define __HIP_PLATFORM_AMD__ 1
