@@ -91,6 +91,23 @@ def test_double_free_detected_not_applied():
         assert nh.canonical_problems() == []
 
 
+def test_free_into_released_superblock_is_a_bad_free():
+    """8 KiB blocks, 2 per 16 KiB superblock: the third malloc finds the
+    first superblock full and detaches it; freeing both of its blocks
+    releases it to the pool (no class any more); freeing one of them again
+    is then counted as a bad free, not a double free, and changes nothing."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16, nctx=8) as h:
+        out = h.trace_serial([8192, 8192, 8192, -1, -2, -1, -3])
+        assert out[0] and out[1] and out[2]
+        assert out[0] >> 14 == out[1] >> 14 != out[2] >> 14
+        st = natimg.NatHeap(h).meta()["stats"]
+        assert int(st[1]) == 1 and int(st[0]) == 0, list(st)
+        r = h.check()
+        assert r["problems"] == 0 and r["live_blocks"] == 0, r
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
 def test_exhaustion_returns_null_then_recovers():
     with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 8, nctx=8, npool=16) as h:
         # 8 superblocks of 2 x 8 KiB blocks
