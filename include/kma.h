@@ -31,7 +31,8 @@ typedef struct kma_config {
 	uint32_t sb_log2;     /* superblock size = 1 << sb_log2 (reference: 12) */
 	uint64_t sblocks;     /* number of superblocks */
 	/* NATIVE only (0 selects the default) */
-	uint32_t nctx;        /* allocation contexts (power of two) */
+	uint32_t nctx;        /* allocation contexts (power of two, 8..65536; 0: the largest keeping one
+				 superblock per class and context within a quarter of the heap, <= 4096) */
 	uint32_t npool;       /* superblock pool shards (power of two, 16..512; 0: min(2 * nctx, 512)) */
 	uint32_t nclasses;    /* entries used in class_size[] */
 	uint32_t class_size[KMA_MAX_CLASSES]; /* ascending block sizes, multiples of 8 */

@@ -52,11 +52,14 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 		return -1;
 	}
 	/* contexts and pool shards are split evenly over the 8 XCDs; each XCD
-	 * has at least one dirty and one clean shard */
+	 * has at least one dirty and one clean shard.  Default contexts: the
+	 * largest power of two that keeps one superblock per class and context
+	 * within a quarter of the heap, at most 4,096 (one per wavefront that
+	 * is resident at once at the allocator kernels' register use) */
 	if (cfg->nctx == 0) {
 		uint64_t want = cfg->sblocks / (4ull * cfg->nclasses);
 		uint32_t n = 8;
-		while ((uint64_t)n * 2 <= want && n < 1024)
+		while ((uint64_t)n * 2 <= want && n < 4096)
 			n *= 2;
 		cfg->nctx = n;
 	}
