@@ -104,3 +104,15 @@ def test_clTree_all_forms():
         f = s.split()
         d = {f[i]: f[i + 1] for i in range(0, len(f) - 1, 2)}
         assert d["edges"] == "20000" and d["links_found"] == "20000" and d["nulls"] == "0", s
+
+
+def test_kma_print_buffer():
+    """-p dumps the first 16 superblocks of the heap as 32-bit words, four to
+    a line with the byte offset (tb_kma.c:296-306).  On kma_create(127) the
+    first word is the heap's size in bytes: 127 superblocks of 4 KiB plus a
+    header of 32 + 8 * 11 bytes (kma.c:79-127) = 520,312 = 0x7f078."""
+    rc, out = run("kma", "-p", "-t", "64,1,1", "-i", "2", "-k", "kma_test_malloc")
+    assert rc == 0, out
+    assert "00000000: 0007f078 00000000 " in out, out[:2000]
+    lines = [l for l in out.splitlines() if re.match(r"^[0-9a-f]{8}: ", l)]
+    assert len(lines) == 16 * 4096 // 16, len(lines)
