@@ -2,9 +2,11 @@
  * tb_kma.c - the `kma` bench program: the reference's allocator benchmark
  * (test/tb_kma.c:222-384) on libkma.so.  For each of the reference's three
  * test kernels (kma_test_malloc, _lowvar, _highvar; kma.cl:424-526) it
- * creates a heap (kma_create(127) as tb_kma.c:234 does, or a native heap
- * with -N), then for every thread configuration and iteration count runs
- * ONE launch of x*y*z work-items, each doing `iters` malloc/write/free
+ * creates a heap (kma_create(127) as tb_kma.c:234 does, or with -N a
+ * native heap: by default 1 GiB of 16 KiB superblocks with 4,096 allocation
+ * contexts, one per wavefront resident at once), then for every thread
+ * configuration and iteration count runs ONE launch of x*y*z work-items,
+ * each doing `iters` malloc/write/free
  * rounds (the kernel's loop, a NULL ends the work-item), and prints one
  * line per run in the reference's format:
  *     <threads> threads <iters> iters: Avg: s.us ( min - max ) Total: s.us
@@ -24,7 +26,8 @@
 
 static unsigned iters = 100, step = 0; /* -i [step,]iters (tb_kma.c:322-347) */
 static int native = 0;                 /* -N: native layout */
-static uint64_t sblocks = 0;           /* -H: superblocks (0: 127, or 16384 native) */
+static uint64_t sblocks = 0;           /* -H: superblocks (0: 127, or 65536 native) */
+static uint32_t nctx = 0;              /* -C: native allocation contexts (0: 4096) */
 static unsigned block = 256;           /* -b: work-group size */
 static unsigned samples = 1;           /* -s: timed launches per point */
 static const char *only = NULL;        /* -k: one kernel */
@@ -82,6 +85,10 @@ static int kma_opts(int i, int argc, char **argv)
 		return -1;
 	if (strncmp(a, "-H", 2) == 0 && parse_uint(argv[i + 1], 1, 1u << 24, &sblocks) == 0)
 		return 1;
+	if (strncmp(a, "-C", 2) == 0 && parse_uint(argv[i + 1], 8, 65536, &v) == 0 && (v & (v - 1)) == 0) {
+		nctx = (uint32_t)v;
+		return 1;
+	}
 	if (strncmp(a, "-b", 2) == 0 && parse_uint(argv[i + 1], 64, 256, &v) == 0 && v % 64 == 0) {
 		block = (unsigned)v;
 		return 1;
@@ -106,7 +113,8 @@ static void usage(void)
 	printf("\t-i [step,]iters:Iteration count for kernel (default: 100)\n");
 	tb_options_print();
 	printf("\t-N:\t\tNative heap layout (default: the reference layout, 4 KiB superblocks)\n");
-	printf("\t-H n:\t\tSuperblocks in the heap (default: 127; native: 16384 of 16 KiB)\n");
+	printf("\t-H n:\t\tSuperblocks in the heap (default: 127; native: 65536 of 16 KiB)\n");
+	printf("\t-C n:\t\tNative allocation contexts, a power of two (default: 4096)\n");
 	printf("\t-b n:\t\tWork-group size, a multiple of 64 up to 256 (default: 256)\n");
 	printf("\t-s n:\t\tTimed launches per point (default: 1)\n");
 	printf("\t-k name:\tRun one kernel: kma_test_malloc, kma_test_malloc_lowvar or\n");
@@ -134,7 +142,8 @@ static kma_heap *make_heap(void)
 	memset(&c, 0, sizeof(c));
 	c.layout = KMA_LAYOUT_NATIVE;
 	c.sb_log2 = 14;
-	c.sblocks = sblocks ? sblocks : 16384u;
+	c.sblocks = sblocks ? sblocks : 65536u;
+	c.nctx = nctx ? nctx : 4096u;
 	return kma_create_ex(&c);
 }
 

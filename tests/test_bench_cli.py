@@ -95,6 +95,8 @@ def test_shipped_thread_configs():
     (["-T", "/nonexistent.cfg"], "Error: file /nonexistent.cfg not found"),
     (["-c"], "no CPU device"),
     (["-b", "100"], "Error: Unknown option: -b"),
+    (["-C", "1000"], "Error: Unknown option: -C"),
+    (["-C", "4"], "Error: Unknown option: -C"),
     (["-k", "kma_test_nothing"], "Error: Unknown option: -k"),
     (["stray"], "Error: Unknown option: stray"),
 ])
