@@ -193,11 +193,12 @@ def bench_graph():
     import kma_amd
 
     edges = kma_amd.graph_generate(GRAPH["nnodes"], GRAPH["nedges"], GRAPH["seed"])
-    out = {"workload": "%d edges over %d node ids (seed %d), 1 GiB heap of 16 KiB superblocks, 1,024 "
-                       "work-groups of 256" % (GRAPH["nedges"], GRAPH["nnodes"], GRAPH["seed"])}
+    out = {"workload": "%d edges over %d node ids (seed %d), 1 GiB heap of 16 KiB superblocks with %d "
+                       "allocation contexts, 1,024 work-groups of 256"
+                       % (GRAPH["nedges"], GRAPH["nnodes"], GRAPH["seed"], NCTX)}
     for mode, name in ((kma_amd.GRAPH_PLAIN, "plain"), (kma_amd.GRAPH_CACHE, "cache"),
                        (kma_amd.GRAPH_COLLECTIVE, "collective")):
-        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 65536) as h:
+        with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS, nctx=NCTX) as h:
             st = kma_amd.graph_build(edges, mode, heap=h, nwg=1024)
         out[name] = {"edges_per_s": st["edges"] / (st["gpu_ms"] * 1e-3), "ms": st["gpu_ms"],
                      "violations": int(st["violations"] + st["bad"] + st["nulls"])}
