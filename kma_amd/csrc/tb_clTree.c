@@ -34,6 +34,7 @@ static const char *dataset = NULL;
 static uint32_t gen_nodes = 0, gen_seed = 1;
 static uint64_t gen_edges = 0;
 static uint64_t sblocks = 0; /* -H: native heap superblocks (0: sized from the data set) */
+static uint32_t nctx = 4096; /* -C: native heap allocation contexts */
 static unsigned samples = 1; /* -s */
 
 static int tree_opts(int i, int argc, char **argv)
@@ -64,6 +65,10 @@ static int tree_opts(int i, int argc, char **argv)
 		sblocks = v;
 		return 1;
 	}
+	if (strncmp(a, "-C", 2) == 0 && v >= 8 && v <= 65536 && (v & (v - 1)) == 0) {
+		nctx = (uint32_t)v;
+		return 1;
+	}
 	if (strncmp(a, "-s", 2) == 0 && v >= 1 && v <= 1000) {
 		samples = (unsigned)v;
 		return 1;
@@ -77,6 +82,7 @@ static void usage(void)
 	tb_options_print();
 	printf("\t-G n,e[,seed]:\tGenerate e edges over n node ids instead of reading a file\n");
 	printf("\t-H n:\t\tSuperblocks of 16 KiB in the heap (default: sized from the data set)\n");
+	printf("\t-C n:\t\tAllocation contexts of the heap, a power of two (default: 4096)\n");
 	printf("\t-s n:\t\tTimed builds per configuration (default: 1)\n");
 	printf("The data file begins with the number of entries, followed by\n");
 	printf("a series of {source, sink} tuples, each on a new line.\n");
@@ -136,7 +142,8 @@ static int run_form(unsigned f, const uint32_t *edges, uint64_t n)
 	printf("-- Executing %s --\n", forms[f].name);
 	uint64_t *us = (uint64_t *)calloc(samples, sizeof(uint64_t));
 	/* nodes <= 2 edges: at most 32 B per node and 16 B per edge live, plus
-	 * one cached 32-B node per lane */
+	 * one cached 32-B node per lane; the heap adds a superblock per context
+	 * for each of the two classes (nodes, links) */
 	uint64_t need = 80ull * n;
 	for (unsigned t = 0; t < tb_opt.wi_entries && us; t++) {
 		uint64_t threads = tb_threads(&tb_opt.wi[t]);
@@ -147,7 +154,7 @@ static int run_form(unsigned f, const uint32_t *edges, uint64_t n)
 		}
 		uint32_t nwg = (uint32_t)((threads + 255) / 256);
 		uint64_t bytes = need + 32ull * 256u * nwg;
-		uint64_t nsb = sblocks ? sblocks : 1024u + 2u * (bytes >> 14);
+		uint64_t nsb = sblocks ? sblocks : 1024u + 2u * (bytes >> 14) + 2u * nctx;
 		if (tb_opt.list) {
 			printf("Threads: %-5" PRIu64 " (%u work-groups of 256)\n", threads, nwg);
 			continue;
@@ -165,6 +172,7 @@ static int run_form(unsigned f, const uint32_t *edges, uint64_t n)
 				c.layout = KMA_LAYOUT_NATIVE;
 				c.sb_log2 = 14;
 				c.sblocks = nsb;
+				c.nctx = nctx;
 				h = kma_create_ex(&c);
 			}
 			int ok = (h || p) && kma_graph_build(h, p, edges, n, forms[f].mode, nwg, 0, &st) == 0;

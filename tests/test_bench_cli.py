@@ -147,6 +147,13 @@ def test_clTree_data_set_errors(tmp_path, content, message):
     assert rc == 255 and message in out, out
 
 
+def test_clTree_contexts_option():
+    rc, out = run("clTree", "-l", "-C", "1000", "-G", "10,10")
+    assert rc == 255 and "Error: Unknown option: -C" in out, out
+    rc, out = run("clTree", "-l", "-C", "256", "-G", "10,10")
+    assert rc == 0, out
+
+
 def test_clTree_missing_file():
     rc, out = run("clTree", "-l", "/nonexistent/data.txt")
     assert rc == 255 and "Error: specified file cannot be opened (/nonexistent/data.txt)" in out, out
