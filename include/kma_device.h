@@ -1304,17 +1304,38 @@ __device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 	} else {
 		order_prior();
 	}
+	if (!live)
+		bump_stat(H, KMA_STAT_BAD_FREE);
+	uint32_t idx = 0;
 	if (live) {
 		uint64_t inoff = off & ((1ull << H->sb_log2) - 1ull);
-		uint32_t idx = (uint32_t)((inoff * H->class_magic[c1 - 1u]) >> 40);
-		uint32_t m = 1u << (idx & 31u);
-		uint32_t old = f_and(bitmap(H, sb) + (idx >> 5), ~m);
-		if (!(old & m)) {
-			bump_stat(H, KMA_STAT_DOUBLE_FREE);
-			live = false;
-		}
-	} else {
-		bump_stat(H, KMA_STAT_BAD_FREE);
+		idx = (uint32_t)((inoff * H->class_magic[c1 - 1u]) >> 40);
+	}
+	/* bit release, one RMW per run of consecutive lanes freeing consecutive
+	 * slots of one bitmap word (as blocks claimed together come back): the
+	 * run's last lane clears the run's bits, its first lane found from the
+	 * ballot of run starts; every lane reads its own bit in the word the
+	 * RMW returned, so a double free is still seen by the lane that made it */
+	const uint32_t l = lane();
+	const uint64_t lv = __ballot(live);
+	const uint32_t psb = __shfl_up(sb, 1u), pidx = __shfl_up(idx, 1u);
+	const bool cont = live && l > 0u && ((lv >> (l - 1u)) & 1u) && psb == sb && pidx + 1u == idx &&
+			  (idx & 31u) != 0u;
+	const uint64_t heads = __ballot(live && !cont), conts = __ballot(cont);
+	const bool last = live && !(l < 63u && ((conts >> (l + 1u)) & 1u));
+	const uint64_t lasts = __ballot(last);
+	uint32_t word = 0;
+	if (last) {
+		const uint32_t first = 63u - (uint32_t)__clzll((unsigned long long)(heads & ((2ull << l) - 1ull)));
+		const uint32_t hi = idx & 31u, lo = hi - (l - first), n = hi - lo + 1u;
+		const uint32_t mask = (n == 32u ? ~0u : ((1u << n) - 1u)) << lo;
+		word = f_and(bitmap(H, sb) + (idx >> 5), ~mask);
+	}
+	const uint32_t src = live ? (uint32_t)__ffsll((unsigned long long)(lasts & ~((1ull << l) - 1ull))) - 1u : l;
+	word = __shfl(word, (int)src);
+	if (live && !(word & (1u << (idx & 31u)))) {
+		bump_stat(H, KMA_STAT_DOUBLE_FREE);
+		live = false;
 	}
 	/* one count update per superblock per wavefront, every superblock at
 	 * once (groups are formed first, with no memory traffic) */

@@ -128,6 +128,7 @@ static inline int __popcll(unsigned long long x) { return __builtin_popcountll(x
 static inline int __ffs(uint32_t x) { return __builtin_ffs((int)x); }
 static inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 static inline int __clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+static inline int __clzll(unsigned long long x) { return x ? __builtin_clzll(x) : 64; }
 
 static inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 static inline uint64_t min(uint64_t a, uint64_t b) { return a < b ? a : b; }
