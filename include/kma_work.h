@@ -106,6 +106,14 @@ int kma_check_wave_claim(kma_heap *h, uint32_t ncases, const uint32_t *cls, cons
 			 const uint32_t *got, const uint32_t *bm_in, uint64_t *res_out,
 			 uint32_t *bm_out, uint32_t *done_out);
 
+/* Test hook with no counterpart in the reference: ONE wavefront frees the
+ * blocks at heap offsets offs[0..n-1] (n <= 64; lane i frees offs[i], 0 = no
+ * request) in a single kma_free call, so that lanes freeing neighbouring
+ * slots of one bitmap word are combined as in a kernel.  Native layout; the
+ * offsets must lie in the heap's data (block offsets as kma_trace_serial
+ * returns them); double frees are counted in the heap's statistics. */
+int kma_wave_free(kma_heap *h, const uint64_t *offs, uint32_t n);
+
 /* Native layout: detach every context's superblock so that fully free
  * superblocks return to the pool (no allocation may be in flight). */
 int kma_trim(kma_heap *h);

@@ -27,7 +27,7 @@ EXPORTS = [
     "kma_create", "kma_create_ex", "kma_destroy", "kma_device_heap",
     "kma_heap_bytes", "kma_get_config", "kma_snapshot", "kma_reset",
     "kma_last_error", "kma_device_ok", "kma_run_workload", "kma_trace_serial",
-    "kma_trim", "kma_atomic_peak", "kma_check_wave_claim", "kma_copy_probe",
+    "kma_trim", "kma_atomic_peak", "kma_check_wave_claim", "kma_wave_free", "kma_copy_probe",
     "kma_stream_create", "kma_stream_destroy", "kma_stream_get_info",
     "kma_stream_trace", "kma_stream_results", "kma_stream_times", "kma_stream_run",
     "kma_stream_heap_config", "kma_stream_autotune",
@@ -251,6 +251,7 @@ def lib():
         L.kma_trace_serial.argtypes = [vp, vp, u64, vp, vp]
         L.kma_trim.argtypes = [vp]
         L.kma_check_wave_claim.argtypes = [vp, u32, vp, vp, vp, vp, vp, vp, vp]
+        L.kma_wave_free.argtypes = [vp, vp, u32]
         L.kma_atomic_peak.argtypes = [u64, u64, u32, ctypes.c_int, u32, u32,
                                       P(ctypes.c_double), P(ctypes.c_double)]
         L.kma_copy_probe.argtypes = [u64, u32, u32, P(ctypes.c_double), P(ctypes.c_double)]
@@ -422,6 +423,12 @@ class Heap:
         _check(lib().kma_trace_serial(self._h, ops.ctypes.data, len(ops), out.ctypes.data, patp),
                "kma_trace_serial")
         return out
+
+    def wave_free(self, offs):
+        """One wavefront frees the blocks at these heap offsets (<= 64, lane i
+        frees offs[i], 0 = none) in a single kma_free call (test hook)."""
+        offs = np.ascontiguousarray(offs, dtype=np.uint64)
+        _check(lib().kma_wave_free(self._h, offs.ctypes.data, len(offs)), "kma_wave_free")
 
     def batch_malloc_det(self, sizes):
         """Deterministic batch malloc (reference layout): (offsets, NULL count,

@@ -503,6 +503,33 @@ done:
 	return rc;
 }
 
+int kma_wave_free(kma_heap *h, const uint64_t *offs, uint32_t n)
+{
+	if (!h || h->cfg.layout != KMA_LAYOUT_NATIVE || !offs || n == 0 || n > 64) {
+		set_err("bad arguments (native heap, 1..64 offsets)");
+		return -1;
+	}
+	for (uint32_t i = 0; i < n; i++)
+		if (offs[i] && (offs[i] < h->nh.off_data || offs[i] >= h->bytes)) {
+			set_err("offset %u outside the heap's data", i);
+			return -1;
+		}
+	uint64_t *d = NULL;
+	if (hipMalloc((void **)&d, n * 8ull) != hipSuccess) {
+		set_err("hipMalloc failed");
+		return -1;
+	}
+	int rc = -1;
+	if (hipMemcpy(d, offs, n * 8ull, hipMemcpyHostToDevice) != hipSuccess)
+		set_err("upload failed");
+	else if (kmad_wave_free(h->dev, d, n) != 0)
+		set_err("free kernel failed");
+	else
+		rc = 0;
+	hipFree(d);
+	return rc;
+}
+
 int kma_check_wave_claim(kma_heap *h, uint32_t n, const uint32_t *cls, const uint32_t *base, const uint32_t *got,
 			 const uint32_t *bm_in, uint64_t *res_out, uint32_t *bm_out, uint32_t *done_out)
 {

@@ -353,6 +353,22 @@ __global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, uint32_t 
 		done_out[cs] = done;
 }
 
+/* One wavefront frees offs[lane] (0: no request) in a single kma_free call. */
+__global__ void __launch_bounds__(64) k_wave_free(kma_dheap *H, const uint64_t *offs, uint32_t n)
+{
+	const uint32_t l = threadIdx.x;
+	const uint64_t o = l < n ? offs[l] : 0u;
+	kma_free(H, o ? (void *)((char *)H + o) : nullptr);
+}
+
+extern "C" int kmad_wave_free(void *heap, const uint64_t *offs, uint32_t n)
+{
+	hipLaunchKernelGGL(k_wave_free, dim3(1), dim3(64), 0, 0, (kma_dheap *)heap, offs, n);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}
+
 extern "C" int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t *cls, const uint32_t *base,
 				     const uint32_t *got, const uint32_t *bm_in, uint64_t *res_out, uint32_t *bm_out,
 				     uint32_t *done_out)

@@ -35,6 +35,7 @@ int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode,
 
 int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses);
 
+int kmad_wave_free(void *heap, const uint64_t *offs, uint32_t n);
 int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t *cls, const uint32_t *base,
 			  const uint32_t *got, const uint32_t *bm_in, uint64_t *res_out,
 			  uint32_t *bm_out, uint32_t *done_out);

@@ -108,6 +108,34 @@ def test_free_into_released_superblock_is_a_bad_free():
         assert natimg.NatHeap(h).canonical_problems() == []
 
 
+def test_wave_free_runs_and_double_frees():
+    """One wavefront frees 64 blocks of consecutive slots: the lanes' bits go
+    back in runs, one RMW per run and bitmap word.  A lane that frees its
+    neighbour's block (a double free inside a run), or a block freed before
+    (at the start of a run), is counted as a double free and changes
+    nothing; every other block is freed, and the heap ends canonical."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 64, nctx=8) as h:
+        stats = lambda: [int(x) for x in natimg.NatHeap(h).meta()["stats"][:2]]
+        offs = h.trace_serial([64] * 64).copy()
+        assert np.all(offs) and np.all(np.diff(offs.astype(np.int64)) == 64)
+        wf = offs.copy()
+        wf[20] = offs[19]                  # lane 20 frees lane 19's block again
+        h.wave_free(wf)
+        assert stats() == [1, 0]
+        h.wave_free([offs[20]])            # the block lane 20 left live
+        assert stats() == [1, 0]
+        r = h.check()
+        assert r["problems"] == 0 and r["live_blocks"] == 0, r
+        again = h.trace_serial([64] * 64).copy()
+        h.wave_free([again[32]])
+        h.wave_free(again)                 # lane 32 starts a run with a freed block
+        assert stats() == [2, 0]
+        r = h.check()
+        assert r["problems"] == 0 and r["live_blocks"] == 0, r
+        h.trim()
+        assert natimg.NatHeap(h).canonical_problems() == []
+
+
 def test_exhaustion_returns_null_then_recovers():
     with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 8, nctx=8, npool=16) as h:
         # 8 superblocks of 2 x 8 KiB blocks
