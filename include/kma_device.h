@@ -1311,35 +1311,7 @@ __device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 		uint64_t inoff = off & ((1ull << H->sb_log2) - 1ull);
 		idx = (uint32_t)((inoff * H->class_magic[c1 - 1u]) >> 40);
 	}
-	/* bit release, one RMW per run of consecutive lanes freeing consecutive
-	 * slots of one bitmap word (as blocks claimed together come back): the
-	 * run's last lane clears the run's bits, its first lane found from the
-	 * ballot of run starts; every lane reads its own bit in the word the
-	 * RMW returned, so a double free is still seen by the lane that made it */
-	const uint32_t l = lane();
-	const uint64_t lv = __ballot(live);
-	const uint32_t psb = __shfl_up(sb, 1u), pidx = __shfl_up(idx, 1u);
-	const bool cont = live && l > 0u && ((lv >> (l - 1u)) & 1u) && psb == sb && pidx + 1u == idx &&
-			  (idx & 31u) != 0u;
-	const uint64_t heads = __ballot(live && !cont), conts = __ballot(cont);
-	const bool last = live && !(l < 63u && ((conts >> (l + 1u)) & 1u));
-	const uint64_t lasts = __ballot(last);
-	uint32_t word = 0;
-	if (last) {
-		const uint32_t first = 63u - (uint32_t)__clzll((unsigned long long)(heads & ((2ull << l) - 1ull)));
-		const uint32_t hi = idx & 31u, lo = hi - (l - first), n = hi - lo + 1u;
-		const uint32_t mask = (n == 32u ? ~0u : ((1u << n) - 1u)) << lo;
-		word = f_and(bitmap(H, sb) + (idx >> 5), ~mask);
-	}
-	const uint32_t src = live ? (uint32_t)__ffsll((unsigned long long)(lasts & ~((1ull << l) - 1ull))) - 1u : l;
-	word = __shfl(word, (int)src);
-	if (live && !(word & (1u << (idx & 31u)))) {
-		bump_stat(H, KMA_STAT_DOUBLE_FREE);
-		live = false;
-	}
-	/* one count update per superblock per wavefront, every superblock at
-	 * once (groups are formed first, with no memory traffic) */
-	uint32_t ctx = ctx_of(H, x);
+	/* superblock groups of the lanes with a block (no memory traffic) */
 	uint64_t left = __ballot(live), grp = 0;
 	while (left) {
 		uint32_t l0 = (uint32_t)__ffsll((unsigned long long)left) - 1u;
@@ -1349,8 +1321,42 @@ __device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 			grp = g;
 		left &= ~g;
 	}
-	if (live && lane() == (uint32_t)__ffsll((unsigned long long)grp) - 1u) {
-		uint32_t n = (uint32_t)__popcll(grp);
+	/* bit release, one RMW per run of a group's members (in lane order)
+	 * freeing consecutive slots of one bitmap word, as blocks claimed
+	 * together come back: the run's last member clears the run's bits, its
+	 * first member found from the ballot of run starts; every lane reads
+	 * its own bit in the word the RMW returned, so a double free is still
+	 * seen by the lane that made it */
+	const uint32_t l = lane();
+	const uint64_t le = (2ull << l) - 1ull; /* lanes 0..l */
+	const uint64_t below = grp & lanemask_lt();
+	const uint32_t pidx = __shfl(idx, (int)(below ? 63u - (uint32_t)__clzll((unsigned long long)below) : l));
+	const bool cont = live && below != 0u && pidx + 1u == idx && (idx & 31u) != 0u;
+	const uint64_t heads = __ballot(live && !cont), conts = __ballot(cont);
+	const uint64_t above = grp & ~le;
+	const bool last = live && !(above && ((conts >> (__ffsll((unsigned long long)above) - 1)) & 1u));
+	const uint64_t lasts = __ballot(last);
+	uint32_t word = 0;
+	if (last) {
+		const uint32_t first = 63u - (uint32_t)__clzll((unsigned long long)(heads & grp & le));
+		const uint32_t span = (uint32_t)__popcll(grp & le & ~((1ull << first) - 1ull)) - 1u;
+		const uint32_t hi = idx & 31u, lo = hi - span, n = span + 1u;
+		const uint32_t mask = (n == 32u ? ~0u : ((1u << n) - 1u)) << lo;
+		word = f_and(bitmap(H, sb) + (idx >> 5), ~mask);
+	}
+	const uint32_t src =
+		live ? (uint32_t)__ffsll((unsigned long long)(lasts & grp & ~lanemask_lt())) - 1u : l;
+	word = __shfl(word, (int)src);
+	if (live && !(word & (1u << (idx & 31u)))) {
+		bump_stat(H, KMA_STAT_DOUBLE_FREE);
+		live = false;
+	}
+	/* one count update per superblock per wavefront (its live members),
+	 * every superblock at once */
+	const uint64_t lv = __ballot(live);
+	uint32_t ctx = ctx_of(H, x);
+	if (live && l == (uint32_t)__ffsll((unsigned long long)(grp & lv)) - 1u) {
+		uint32_t n = (uint32_t)__popcll(grp & lv);
 		uint64_t ns = fsub(state(H, sb), (uint64_t)n) - n;
 		if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
 			try_release(H, sb, ns, x, ctx);
