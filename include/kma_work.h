@@ -405,6 +405,31 @@ int kma_heap_check(kma_heap *h, kma_heap_report *out);
  * resident.  Returns the number of NULLs, or -1. */
 int64_t kma_batch_malloc_det(kma_heap *h, const uint32_t *sizes, uint64_t n, uint64_t *out, double *ms);
 
+/* Allocation-free form of kma_batch_malloc_det.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc / hipFree and no allocating library
+ * routine, and does not wait: the kernels are enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) and the call returns.
+ *   sizes  DEVICE array of n request sizes (input)
+ *   out    DEVICE array of n heap offsets (output; 0 = NULL), as
+ *          kma_batch_malloc_det returns them
+ *   status DEVICE array of 2 u64 (output): [0] the number of NULLs, [1] the
+ *          requests refused for falling in a 1-slot class (when nonzero the
+ *          heap image is undefined and must be reset, as for the blocking form)
+ * Same heap precondition as kma_batch_malloc_det.  Intermediate results
+ * live in a static device scratch of kma_det_noalloc_scratch_bytes() bytes
+ * per GPU (about 8.9 MiB, held until the process ends; its address on a
+ * GPU is resolved when the first reference-layout heap is created there or
+ * at the first call).  Every call on one GPU shares that scratch, so calls
+ * that overlap on one GPU (other streams, other threads) race; calls in
+ * order on one stream do not.  -1 with kma_last_error() for n == 0,
+ * n > KMA_DET_NOALLOC_MAX, a native-layout heap or a failed launch. */
+#define KMA_DET_NOALLOC_MAX 524288u
+int kma_batch_malloc_det_noalloc(kma_heap *h, const uint32_t *sizes, uint64_t n, uint64_t *out,
+				 uint64_t *status, void *stream);
+uint64_t kma_det_noalloc_scratch_bytes(void);
+
 /* Free the blocks of the preceding kma_batch_malloc_det batch (offs = its
  * out array, n its length) with the result of free(offs[0]), ...,
  * free(offs[n-1]) run one at a time in index order (kma.cl:305-373). */

@@ -37,7 +37,9 @@ EXPORTS = [
     "kma_list_run_grow", "kma_list_run_get", "kma_list_clear", "kma_list_pages", "kma_tree_run",
     "kma_batch_malloc_det", "kma_batch_free_det", "kma_serve_run", "kma_serve_trace", "kma_serve_lengths",
     "kma_graph_generate", "kma_graph_build", "kma_image_check", "kma_heap_check",
+    "kma_batch_malloc_det_noalloc", "kma_det_noalloc_scratch_bytes",
 ]
+DET_NOALLOC_MAX = 524288
 GRAPH_PLAIN, GRAPH_CACHE, GRAPH_COLLECTIVE = 0, 1, 2
 TABLE_POW2, TABLE_HALF = 0, 1
 TUNE_MAX = 64
@@ -301,6 +303,9 @@ def lib():
         L.kma_batch_malloc_det.argtypes = [vp, vp, u64, vp, P(ctypes.c_double)]
         L.kma_batch_malloc_det.restype = i64
         L.kma_batch_free_det.argtypes = [vp, vp, u64, P(ctypes.c_double)]
+        L.kma_batch_malloc_det_noalloc.argtypes = [vp, vp, u64, vp, vp, vp]
+        L.kma_det_noalloc_scratch_bytes.argtypes = []
+        L.kma_det_noalloc_scratch_bytes.restype = u64
         L.kma_serve_run.argtypes = [vp, P(ServeSpec), vp, u64, vp, u64, P(ServeStats)]
         L.kma_serve_trace.argtypes = [P(ServeSpec), vp, vp, u64]
         L.kma_serve_trace.restype = i64
@@ -441,6 +446,14 @@ class Heap:
         if nulls < 0:
             raise KmaError("kma_batch_malloc_det: " + last_error())
         return out, nulls, ms.value
+
+    def batch_malloc_det_noalloc(self, d_sizes, n, d_out, d_status, stream=None):
+        """Allocation-free deterministic batch (kma_batch_malloc_det_noalloc):
+        every argument is a raw device address (int) owned by the caller; the
+        kernels are enqueued on `stream` (a hipStream_t as int, None = null
+        stream) and the call returns without waiting."""
+        _check(lib().kma_batch_malloc_det_noalloc(self._h, d_sizes, n, d_out, d_status, stream),
+               "kma_batch_malloc_det_noalloc")
 
     def batch_free_det(self, offs):
         """Free the preceding deterministic batch in index order; device ms."""

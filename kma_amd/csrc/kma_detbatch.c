@@ -81,19 +81,14 @@ int64_t kma_batch_malloc_det(kma_heap *h, const uint32_t *sizes, uint64_t n, uin
 		set_err("deterministic batch: device allocation failed");
 		goto done;
 	}
-	uint32_t *nev = small, *npop = small + 1, *err = small + 2, *coff = small + 16;
 	if (hipMemcpy(ds, sizes, n * 4, hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemset(small, 0, 64 * 4) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
 		set_err("deterministic batch: upload failed");
 		goto done;
 	}
+	kmad_det_bufs bufs = {cls, rank, cnt, tot, evcnt, small, table, evc, evq, popped};
 	hipEventRecord(e0, 0);
-	if (kmad_det_class(h->dev, ds, n, cls, cnt, err) || kmad_det_scan(cnt, nb, DCLS, tot) ||
-	    kmad_det_rank(h->dev, cls, n, cnt, rank, evcnt) || kmad_det_scan(evcnt, nb, 1, nev) ||
-	    kmad_det_coff(h->dev, tot, coff) ||
-	    kmad_det_events(h->dev, cls, rank, n, evcnt, coff, table, evc, evq) ||
-	    kmad_det_pops(h->dev, nev, popped, npop) || kmad_det_refill(h->dev, n, npop, popped, evc, evq, tot) ||
-	    kmad_det_claim(h->dev, cls, rank, n, coff, table, npop, popped, dout)) {
+	if (kmad_det_malloc(h->dev, ds, n, &bufs, dout, NULL, NULL)) {
 		set_err("deterministic batch: launch failed");
 		goto done;
 	}
@@ -128,6 +123,33 @@ done:
 		hipEventDestroy(e1);
 	dfree(&b);
 	return rc;
+}
+
+int kma_batch_malloc_det_noalloc(kma_heap *h, const uint32_t *sizes, uint64_t n, uint64_t *out, uint64_t *status,
+				 void *stream)
+{
+	if (check_heap(h) != 0)
+		return -1;
+	if (n == 0 || n > KMA_DET_NOALLOC_MAX) {
+		set_err("allocation-free batch: n = %llu outside 1..%u (the static scratch holds %u requests)",
+			(unsigned long long)n, KMA_DET_NOALLOC_MAX, KMA_DET_NOALLOC_MAX);
+		return -1;
+	}
+	if (!sizes || !out || !status) {
+		set_err("bad arguments");
+		return -1;
+	}
+	int rc = kmad_det_malloc_noalloc(h->dev, sizes, n, out, (unsigned long long *)status, stream);
+	if (rc != 0) {
+		set_err("allocation-free batch: launch failed: %d (%s)", rc, hipGetErrorString((hipError_t)(-rc)));
+		return -1;
+	}
+	return 0;
+}
+
+uint64_t kma_det_noalloc_scratch_bytes(void)
+{
+	return kmad_det_scratch_bytes();
 }
 
 int kma_batch_free_det(kma_heap *h, const uint64_t *offs, uint64_t n, double *ms)

@@ -31,9 +31,11 @@
  * enqueued in index order of those frees, again on one lane.
  */
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 
 #include "kma_launch.h"
 #include "kma_ref_device.h"
+#include "kma_work.h"
 
 #define KMA_CHECK(x)                                   \
 	do {                                           \
@@ -242,10 +244,11 @@ __global__ void __launch_bounds__(DB) k_det_refill(clheap *heap, const uint32_t 
 		*cls_head(v, c) = (uint64_t)(uintptr_t)sb;
 }
 
-/* block, ticket and bit of every request */
+/* block, ticket and bit of every request; NULL results counted in *nulls
+ * when it is given (one atomic per wavefront) */
 __global__ void __launch_bounds__(DB) k_det_claim(clheap *heap, const int8_t *cls, const uint32_t *rank, uint64_t n,
 						  const uint32_t *coff, const uint32_t *table, const uint32_t *npop,
-						  const uint32_t *popped, uint64_t *out)
+						  const uint32_t *popped, uint64_t *out, unsigned long long *nulls)
 {
 	view v = make_view(heap);
 	uint64_t i = (uint64_t)blockIdx.x * DB + threadIdx.x;
@@ -266,6 +269,31 @@ __global__ void __launch_bounds__(DB) k_det_claim(clheap *heap, const int8_t *cl
 		}
 	}
 	out[i] = res;
+	if (nulls) {
+		/* lanes past n have returned; lane 0 of a wavefront with any
+		 * active lane is active (it has the wavefront's lowest index) */
+		uint64_t m = __ballot(res == 0);
+		if (__lane_id() == 0 && m)
+			atomicAdd(nulls, (unsigned long long)__popcll(m));
+	}
+}
+
+/* zero `n` words (the allocation-free entry point clears its scratch
+ * counters with a kernel, not a memset, so the sequence can be captured) */
+__global__ void k_det_zero(uint32_t *p, uint32_t n, unsigned long long *q, uint32_t nq)
+{
+	uint32_t t = threadIdx.x;
+	if (t < n)
+		p[t] = 0;
+	if (t < nq)
+		q[t] = 0;
+}
+
+/* refused requests (1-slot classes) into the caller's status word */
+__global__ void k_det_status(const uint32_t *small, unsigned long long *status)
+{
+	if (threadIdx.x == 0)
+		status[1] = small[2];
 }
 
 /* ---- free ---- */
@@ -370,68 +398,9 @@ static inline uint32_t nblk(uint64_t n)
 }
 
 /* ---- launchers (all on the null stream, synchronous) ---- */
-extern "C" int kmad_det_class(void *heap, const uint32_t *sizes, uint64_t n, int8_t *cls, uint32_t *cnt,
-			      uint32_t *err)
-{
-	hipLaunchKernelGGL(k_det_class, dim3(nblk(n)), dim3(DB), 0, 0, (clheap *)heap, sizes, n, cls, cnt, err);
-	KMA_CHECK(hipGetLastError());
-	return 0;
-}
-
 extern "C" int kmad_det_scan(uint32_t *tab, uint64_t nrows, uint32_t ncol, uint32_t *tot)
 {
 	hipLaunchKernelGGL(k_det_scan, dim3(ncol), dim3(DB), 0, 0, tab, nrows, ncol, tot);
-	KMA_CHECK(hipGetLastError());
-	return 0;
-}
-
-extern "C" int kmad_det_rank(void *heap, const int8_t *cls, uint64_t n, const uint32_t *off, uint32_t *rank,
-			     uint32_t *evcnt)
-{
-	hipLaunchKernelGGL(k_det_rank, dim3(nblk(n)), dim3(DB), 0, 0, (clheap *)heap, cls, n, off, rank, evcnt);
-	KMA_CHECK(hipGetLastError());
-	return 0;
-}
-
-extern "C" int kmad_det_events(void *heap, const int8_t *cls, const uint32_t *rank, uint64_t n, const uint32_t *evoff,
-			       const uint32_t *coff, uint32_t *table, uint32_t *evc, uint32_t *evq)
-{
-	hipLaunchKernelGGL(k_det_events, dim3(nblk(n)), dim3(DB), 0, 0, (clheap *)heap, cls, rank, n, evoff, coff, table,
-			   evc, evq);
-	KMA_CHECK(hipGetLastError());
-	return 0;
-}
-
-extern "C" int kmad_det_coff(void *heap, const uint32_t *tot, uint32_t *coff)
-{
-	hipLaunchKernelGGL(k_det_coff, dim3(1), dim3(64), 0, 0, (clheap *)heap, tot, coff);
-	KMA_CHECK(hipGetLastError());
-	return 0;
-}
-
-extern "C" int kmad_det_pops(void *heap, const uint32_t *nev, uint32_t *popped, uint32_t *npop)
-{
-	hipLaunchKernelGGL(k_det_pops, dim3(1), dim3(64), 0, 0, (clheap *)heap, nev, popped, npop);
-	KMA_CHECK(hipGetLastError());
-	return 0;
-}
-
-/* maxev bounds the number of refill events (grid size only) */
-extern "C" int kmad_det_refill(void *heap, uint64_t maxev, const uint32_t *npop, const uint32_t *popped,
-			       const uint32_t *evc, const uint32_t *evq, const uint32_t *tot)
-{
-	if (maxev)
-		hipLaunchKernelGGL(k_det_refill, dim3(nblk(maxev)), dim3(DB), 0, 0, (clheap *)heap, npop, popped, evc,
-				   evq, tot);
-	KMA_CHECK(hipGetLastError());
-	return 0;
-}
-
-extern "C" int kmad_det_claim(void *heap, const int8_t *cls, const uint32_t *rank, uint64_t n, const uint32_t *coff,
-			      const uint32_t *table, const uint32_t *npop, const uint32_t *popped, uint64_t *out)
-{
-	hipLaunchKernelGGL(k_det_claim, dim3(nblk(n)), dim3(DB), 0, 0, (clheap *)heap, cls, rank, n, coff, table, npop,
-			   popped, out);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }
@@ -463,6 +432,123 @@ extern "C" int kmad_detf_order(void *heap, const uint64_t *offs, uint64_t n, con
 extern "C" int kmad_detf_enqueue(void *heap, const uint32_t *order, const uint32_t *ne)
 {
 	hipLaunchKernelGGL(k_detf_enqueue, dim3(1), dim3(64), 0, 0, (clheap *)heap, order, ne);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
+/* The whole malloc batch on `stream`: every kernel of kma_batch_malloc_det in
+ * order.  b->small[2] (refusals) must be zero beforehand; *nulls (optional)
+ * gets the NULL results added. */
+extern "C" int kmad_det_malloc(void *heap, const uint32_t *sizes, uint64_t n, const kmad_det_bufs *b, uint64_t *out,
+			       unsigned long long *nulls, void *stream)
+{
+	hipStream_t s = (hipStream_t)stream;
+	clheap *hp = (clheap *)heap;
+	uint64_t nb = (n + DB - 1) / DB;
+	uint32_t *nev = b->small, *npop = b->small + 1, *err = b->small + 2, *coff = b->small + 16;
+	hipLaunchKernelGGL(k_det_class, dim3(nblk(n)), dim3(DB), 0, s, hp, sizes, n, b->cls, b->cnt, err);
+	hipLaunchKernelGGL(k_det_scan, dim3(DCLS), dim3(DB), 0, s, b->cnt, nb, DCLS, b->tot);
+	hipLaunchKernelGGL(k_det_rank, dim3(nblk(n)), dim3(DB), 0, s, hp, b->cls, n, b->cnt, b->rank, b->evcnt);
+	hipLaunchKernelGGL(k_det_scan, dim3(1), dim3(DB), 0, s, b->evcnt, nb, 1u, nev);
+	hipLaunchKernelGGL(k_det_coff, dim3(1), dim3(64), 0, s, hp, b->tot, coff);
+	hipLaunchKernelGGL(k_det_events, dim3(nblk(n)), dim3(DB), 0, s, hp, b->cls, b->rank, n, b->evcnt, coff,
+			   b->table, b->evc, b->evq);
+	hipLaunchKernelGGL(k_det_pops, dim3(1), dim3(64), 0, s, hp, nev, b->popped, npop);
+	hipLaunchKernelGGL(k_det_refill, dim3(nblk(n)), dim3(DB), 0, s, hp, npop, b->popped, b->evc, b->evq, b->tot);
+	hipLaunchKernelGGL(k_det_claim, dim3(nblk(n)), dim3(DB), 0, s, hp, b->cls, b->rank, n, coff, b->table, npop,
+			   b->popped, out, nulls);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Allocation-free entry point                                          */
+/* ------------------------------------------------------------------ */
+/* Scratch of kma_batch_malloc_det_noalloc, one copy per GPU, held until the
+ * process ends.  Carved as (bytes, for n <= KMA_DET_NOALLOC_MAX = 2^19):
+ *   cls n | rank 4n | table 4(n + DCLS) | evc 4n | evq 4n |
+ *   popped 4 * 65536 (a reference-layout queue holds < 65,536 superblocks) |
+ *   cnt 4 * DCLS * n/256 | evcnt 4 * n/256 | tot 4 * DCLS | small 4 * 64 */
+#define NA_N ((uint64_t)KMA_DET_NOALLOC_MAX)
+#define NA_NB (NA_N / DB)
+#define NA_CLS 0ull
+#define NA_RANK (NA_CLS + NA_N)
+#define NA_TABLE (NA_RANK + 4 * NA_N)
+#define NA_EVC (NA_TABLE + 4 * (NA_N + DCLS))
+#define NA_EVQ (NA_EVC + 4 * NA_N)
+#define NA_POPPED (NA_EVQ + 4 * NA_N)
+#define NA_CNT (NA_POPPED + 4 * 65536ull)
+#define NA_EVCNT (NA_CNT + 4ull * DCLS * NA_NB)
+#define NA_TOT (NA_EVCNT + 4 * NA_NB)
+#define NA_SMALL (NA_TOT + 4 * DCLS)
+#define NA_BYTES (NA_SMALL + 4 * 64)
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+/* The device address of `scratch` on each GPU (the array's name in host
+ * code is only a host-side handle; each GPU holds its own copy at its own
+ * address).  Resolved once per device with that device current. */
+#define NA_MAXDEV 64
+static void *g_scratch_dev[NA_MAXDEV];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+extern "C" int kmad_det_scratch_setup(void **out)
+{
+	int dev = -1;
+	KMA_CHECK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= NA_MAXDEV)
+		return -(int)hipErrorInvalidDevice;
+	pthread_mutex_lock(&g_scratch_mu);
+	int rc = 0;
+	if (!g_scratch_dev[dev]) {
+		void *p = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			rc = -(int)(e != hipSuccess ? e : hipErrorInvalidSymbol);
+		else
+			g_scratch_dev[dev] = p;
+	}
+	if (out)
+		*out = g_scratch_dev[dev];
+	pthread_mutex_unlock(&g_scratch_mu);
+	return rc;
+}
+
+extern "C" uint64_t kmad_det_scratch_bytes(void)
+{
+	return NA_BYTES;
+}
+
+extern "C" int kmad_det_malloc_noalloc(void *heap, const uint32_t *sizes, uint64_t n, uint64_t *out,
+				       unsigned long long *status, void *stream)
+{
+	if (n == 0 || n > NA_N)
+		return -(int)hipErrorInvalidValue;
+	void *base = nullptr;
+	int rc = kmad_det_scratch_setup(&base);
+	if (rc != 0)
+		return rc;
+	unsigned char *sp = (unsigned char *)base;
+	kmad_det_bufs b;
+	b.cls = (int8_t *)(sp + NA_CLS);
+	b.rank = (uint32_t *)(sp + NA_RANK);
+	b.table = (uint32_t *)(sp + NA_TABLE);
+	b.evc = (uint32_t *)(sp + NA_EVC);
+	b.evq = (uint32_t *)(sp + NA_EVQ);
+	b.popped = (uint32_t *)(sp + NA_POPPED);
+	b.cnt = (uint32_t *)(sp + NA_CNT);
+	b.evcnt = (uint32_t *)(sp + NA_EVCNT);
+	b.tot = (uint32_t *)(sp + NA_TOT);
+	b.small = (uint32_t *)(sp + NA_SMALL);
+	hipStream_t s = (hipStream_t)stream;
+	/* the previous call's counters are still there: clear the small words
+	 * and the caller's status with a kernel */
+	hipLaunchKernelGGL(k_det_zero, dim3(1), dim3(64), 0, s, b.small, 64u, status, 2u);
+	KMA_CHECK(hipGetLastError());
+	rc = kmad_det_malloc(heap, sizes, n, &b, out, status, stream);
+	if (rc != 0)
+		return rc;
+	hipLaunchKernelGGL(k_det_status, dim3(1), dim3(64), 0, s, b.small, status);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }

@@ -75,6 +75,15 @@ static kma_heap *create_reference(const kma_config *cfg)
 		free(h);
 		return NULL;
 	}
+	/* resolve this GPU's scratch address for the allocation-free batch
+	 * entry point now, so that its calls need not */
+	int rc = kmad_det_scratch_setup(NULL);
+	if (rc != 0) {
+		set_err("scratch address of the allocation-free batch: %s", hipGetErrorString((hipError_t)(-rc)));
+		hipFree(h->dev);
+		free(h);
+		return NULL;
+	}
 	return h;
 }
 

@@ -115,21 +115,25 @@ int kmad_serve_max_groups(uint32_t *out);
 int kmad_serve_launch(const kma_serve_args *a, uint32_t nwg, void *stream);
 
 /* Deterministic batch mode, reference layout (kma_detbatch.hip). */
-int kmad_det_class(void *heap, const uint32_t *d_sizes, uint64_t n, int8_t *d_cls, uint32_t *d_cnt,
-		   uint32_t *d_err);
 int kmad_det_scan(uint32_t *d_tab, uint64_t nrows, uint32_t ncol, uint32_t *d_tot);
-int kmad_det_rank(void *heap, const int8_t *d_cls, uint64_t n, const uint32_t *d_off, uint32_t *d_rank,
-		  uint32_t *d_evcnt);
-int kmad_det_coff(void *heap, const uint32_t *d_tot, uint32_t *d_coff);
-int kmad_det_events(void *heap, const int8_t *d_cls, const uint32_t *d_rank, uint64_t n,
-		    const uint32_t *d_evoff, const uint32_t *d_coff, uint32_t *d_table, uint32_t *d_evc,
-		    uint32_t *d_evq);
-int kmad_det_pops(void *heap, const uint32_t *d_nev, uint32_t *d_popped, uint32_t *d_npop);
-int kmad_det_refill(void *heap, uint64_t maxev, const uint32_t *d_npop, const uint32_t *d_popped,
-		    const uint32_t *d_evc, const uint32_t *d_evq, const uint32_t *d_tot);
-int kmad_det_claim(void *heap, const int8_t *d_cls, const uint32_t *d_rank, uint64_t n,
-		   const uint32_t *d_coff, const uint32_t *d_table, const uint32_t *d_npop,
-		   const uint32_t *d_popped, uint64_t *d_out);
+/* Buffers of one deterministic malloc batch of n requests (nb = ceil(n/256)):
+ * cls[n], rank[n], cnt[nb*16], tot[16], evcnt[nb], small[64] ([0] refill
+ * events, [1] successful pops, [2] refused requests, [16..32) class offsets),
+ * table[n+16], evc[n], evq[n], popped[min(n, 65536)]. */
+typedef struct kmad_det_bufs {
+	int8_t *cls;
+	uint32_t *rank, *cnt, *tot, *evcnt, *small, *table, *evc, *evq, *popped;
+} kmad_det_bufs;
+/* Every kernel of one malloc batch, in order, on `stream`. */
+int kmad_det_malloc(void *heap, const uint32_t *d_sizes, uint64_t n, const kmad_det_bufs *b, uint64_t *d_out,
+		    unsigned long long *d_nulls, void *stream);
+/* Allocation-free form: the buffers are carved from the library's static
+ * device scratch (one per GPU).  kmad_det_scratch_setup resolves that GPU's
+ * scratch address (current device) once; *out (optional) receives it. */
+int kmad_det_scratch_setup(void **out);
+uint64_t kmad_det_scratch_bytes(void);
+int kmad_det_malloc_noalloc(void *heap, const uint32_t *d_sizes, uint64_t n, uint64_t *d_out,
+			    unsigned long long *d_status, void *stream);
 int kmad_detf_min(void *heap, const uint64_t *d_offs, uint64_t n, uint32_t *d_minblk);
 int kmad_detf_words(void *heap, const uint64_t *d_offs, uint64_t n, const uint32_t *d_minblk,
 		    uint32_t *d_evcnt, uint32_t *d_last_sb);
