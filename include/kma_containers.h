@@ -17,10 +17,10 @@ extern "C" {
 /* ---- bump allocator ---- */
 typedef struct kma_pma kma_pma;
 
-/* pma_create(dev, ctx, cq, prg, bytes), pma.h:93-94 / pma.c:188-226:
+/* pma_create(dev, ctx, cq, prg, bytes), pma.h:45-46 / pma.c:79-127:
  * `bytes` of data (the header is extra), sizes rounded up to `align` (a
  * power of two; 1 = unaligned like the reference).  NULL for bytes == 0
- * (pma.c:198-199). */
+ * (pma.c:89-90). */
 kma_pma *kma_pma_create(uint64_t bytes, uint32_t align);
 void kma_pma_destroy(kma_pma *p);
 /* Device address to pass to kernels (kma_pma_dheap *, kma_pma.h). */

@@ -8,10 +8,11 @@
  * request still advances the head (it is never rolled back).  Differences,
  * none of which changes a result the reference defines:
  *   - the head is 64-bit, so failed requests cannot wrap it (the
- *     reference's u32 head wraps after 4 GiB of requests, pma.h:104);
+ *     reference's u32 head, pma.h:56, wraps after 4 GiB of requests,
+ *     pma.cl:37);
  *   - `bytes` is the data capacity, the header is outside it (the
- *     reference counts the header in `bytes` and can overrun by its size,
- *     pma.c:146);
+ *     reference counts the header in `bytes`, pma.c:37, and checks the
+ *     offset against it, pma.cl:38, so it can overrun by the header's size);
  *   - sizes are rounded up to the heap's alignment (1 = the reference's
  *     unaligned behaviour);
  *   - the lanes of a wavefront that call together are served by ONE atomic

@@ -56,7 +56,7 @@ struct kma_list {
 /* ---------------- bump allocator ---------------- */
 kma_pma *kma_pma_create(uint64_t bytes, uint32_t align)
 {
-	if (bytes == 0) { /* pma.c:198-199 */
+	if (bytes == 0) { /* pma.c:89-90 */
 		set_err("bytes == 0");
 		return NULL;
 	}

@@ -78,14 +78,16 @@ static int tree_opts(int i, int argc, char **argv)
 
 static void usage(void)
 {
-	printf("Usage: clTree [options] [data file]\n");
+	printf("Usage: clTree [options] [edge file]\n");
+	printf("Builds a graph's adjacency lists in an insert-only tree with each\n");
+	printf("allocation form, one build per launch shape.\n");
 	tb_options_print();
-	printf("\t-G n,e[,seed]:\tGenerate e edges over n node ids instead of reading a file\n");
-	printf("\t-H n:\t\tSuperblocks of 16 KiB in the heap (default: sized from the data set)\n");
-	printf("\t-C n:\t\tAllocation contexts of the heap, a power of two (default: 4096)\n");
-	printf("\t-s n:\t\tTimed builds per configuration (default: 1)\n");
-	printf("The data file begins with the number of entries, followed by\n");
-	printf("a series of {source, sink} tuples, each on a new line.\n");
+	printf("  -G n,e[,s]   generate e edges over n node ids (seed s) instead of a file\n");
+	printf("  -H n         superblocks of 16 KiB per heap (default: sized from the edges)\n");
+	printf("  -C n         allocation contexts per heap, a power of two (default 4096)\n");
+	printf("  -s n         timed builds per launch shape (default 1)\n");
+	printf("\nAn edge file holds the edge count on its first line, then one\n");
+	printf("{source,sink} pair per line.\n");
 }
 
 /* The data set file (tb_clTree.c:350-389); *edges = 2 * n u32 on success. */

@@ -99,15 +99,14 @@ int tb_options_read(int argc, char **argv, tb_extra_fn extra)
 
 void tb_options_print(void)
 {
-	printf("\t-c:\t\tExecute on CPU (refused: GPU only)\n");
-	printf("\t-p:\t\tPrint buffer(s)\n");
-	printf("\t-g:\t\tPrint each run's device counters\n");
-	printf("\t-l:\t\tList the runs, do not execute them\n");
-	printf("\t-t x,y,z:\tProvide a single thread-configuration\n\n");
-	printf("\t-T [file]:\tProvide a file with thread-configurations\n");
-	printf("Thread configuration files are {x, y, z} tuples, always in 3 dimensions.\n");
-	printf("The first line contains the amount of tuples. If no argument is given\n");
-	printf("the program starts with a single work-item.\n");
+	printf("  -c           run on the host CPU (not available: this build has no CPU device)\n");
+	printf("  -p           hex-dump the first superblocks of each heap after its runs\n");
+	printf("  -g           after each run, print the device counters and any give-ups\n");
+	printf("  -l           only list the planned runs; the GPU is not opened\n");
+	printf("  -t x,y,z     run one launch shape: x*y*z work-items\n");
+	printf("  -T file      run every launch shape listed in `file`\n");
+	printf("\nA shape file starts with the number of shapes, followed by one {x,y,z}\n");
+	printf("line per shape.  Without -t or -T a single work-item runs.\n");
 }
 
 uint64_t tb_threads(const tb_tuple *t)
