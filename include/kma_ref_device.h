@@ -28,6 +28,18 @@
  *     64-wide wavefront the spinner can be the lane that holds the block).
  *  4. sizes 1..3 map to class 0 and size 0 returns NULL (the reference leaves
  *     the class uninitialised, kma.cl:129-135).
+ *  5. a malloc whose class has no superblock attached while the free queue
+ *     is empty returns NULL after plain loads instead of taking the class
+ *     head's POISON lock to find the queue empty (same image, but without it
+ *     every failing malloc of an exhausted heap queues on one word).
+ *
+ * Every wait is bounded.  Calls that may fail (reserve, the dequeue of a
+ * refill) give up at a per-call watchdog or at a deadline the host arms in
+ * the heap's stats trailer, and return NULL; calls that hold a reservation
+ * or a released superblock (claim, free, enqueue) are lock-free and never
+ * give up on a consistent heap.  A claim that still finds no free bit
+ * returns its reserved slot.  Each give-up is counted in the trailer (256
+ * bytes after the image, not part of the reference's layout).
  */
 #ifndef KMA_REF_DEVICE_H
 #define KMA_REF_DEVICE_H
@@ -41,24 +53,23 @@ namespace kma_ref {
 
 static constexpr uint64_t POISON = 1u;
 /* Every wait in this file gives up after this many ticks of the constant
- * 100 MHz wall clock; reaching it means the heap is corrupt or starved, and
- * the call returns NULL instead of spinning on. */
-static constexpr uint64_t WAIT_TICKS = 400000000ull; /* 4 s */
+ * 100 MHz wall clock; reaching it means the heap is corrupt or starved. */
+#ifndef KMA_REF_WAIT_TICKS
+#define KMA_REF_WAIT_TICKS 400000000ull /* 4 s */
+#endif
+static constexpr uint64_t WAIT_TICKS = KMA_REF_WAIT_TICKS;
 
-struct waiter {
-	uint64_t t0 = 0;
-	uint32_t n = 0;
-	__device__ __forceinline__ bool expired()
-	{
-		if ((++n & 255u) != 0)
-			return false;
-		uint64_t t = wall_clock64();
-		if (t0 == 0) {
-			t0 = t;
-			return false;
-		}
-		return t - t0 > WAIT_TICKS;
-	}
+/* Stats trailer: 256 bytes right after the heap image (at the header's
+ * `bytes`, rounded up to 64), outside the reference's layout, so images stay
+ * byte-exact.  u64 words: */
+enum {
+	T_DEADLINE = 0, /* wall-clock tick after which waits give up (0: none) */
+	T_RESERVE = 1,  /* reserve gave up on its watchdog: malloc -> NULL */
+	T_DLHIT = 2,    /* reserve gave up at the deadline: malloc -> NULL */
+	T_CLAIM = 3,    /* claim found no free bit; its reserved slot was returned */
+	T_DEQUEUE = 4,  /* dequeue gave up: the refill saw an empty queue */
+	T_ENQUEUE = 5,  /* enqueue gave up (corrupt queue): a superblock is lost */
+	T_NWORDS = 8
 };
 
 /* ---- agent-scope atomics ---- */
@@ -85,9 +96,9 @@ __device__ __forceinline__ T xchg(T *p, T v)
 	return __hip_atomic_exchange(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* Exponential back-off between retries of a spin on a shared word, so that
- * thousands of spinning lanes do not starve the lane they wait for of the
- * word's atomic unit. */
+/* Exponential back-off between retries of a spin or a failed CAS on a
+ * shared word, so that thousands of spinning lanes do not starve the lane
+ * they wait for of the word's atomic unit. */
 __device__ __forceinline__ void backoff(uint32_t &k)
 {
 	for (uint32_t i = 0; i < k; i++)
@@ -97,8 +108,9 @@ __device__ __forceinline__ void backoff(uint32_t &k)
 
 /* ---- layout ---- */
 struct view {
-	char *h;      /* heap header */
-	char *sb0;    /* first superblock == free.base */
+	char *h;       /* heap header */
+	char *sb0;     /* first superblock == free.base */
+	uint64_t *trl; /* stats trailer */
 	uint32_t log2;
 	uint32_t sbsz;
 	uint32_t buckets;
@@ -112,8 +124,43 @@ __device__ __forceinline__ view make_view(clheap *heap)
 	v.sbsz = 1u << v.log2;
 	v.buckets = v.log2 - 1u;
 	v.sb0 = v.h + 32 + 8u * v.buckets;
+	uint64_t bytes = *reinterpret_cast<const uint64_t *>(v.h);
+	v.trl = reinterpret_cast<uint64_t *>(v.h + ((bytes + 63u) & ~63ull));
 	return v;
 }
+
+__device__ __forceinline__ void count(const view &v, int k)
+{
+	__hip_atomic_fetch_add(v.trl + k, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* true once the host-armed deadline has passed */
+__device__ __forceinline__ bool past_deadline(const view &v)
+{
+	uint64_t dl = __hip_atomic_load(v.trl + T_DEADLINE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return dl != 0 && wall_clock64() > dl;
+}
+
+/* Wait bound of one call: the per-call watchdog and, where the caller may
+ * give up (use_deadline), the heap's deadline; both read every 256 rounds.
+ * expired(): 0 keep waiting, 1 watchdog, 2 deadline. */
+struct waiter {
+	uint64_t t0 = 0;
+	uint32_t n = 0;
+	__device__ __forceinline__ int expired(const view &v, bool use_deadline)
+	{
+		if ((++n & 255u) != 0)
+			return 0;
+		if (use_deadline && past_deadline(v))
+			return 2;
+		uint64_t t = wall_clock64();
+		if (t0 == 0) {
+			t0 = t;
+			return 0;
+		}
+		return t - t0 > WAIT_TICKS ? 1 : 0;
+	}
+};
 
 __device__ __forceinline__ uint32_t *q_head(const view &v) { return reinterpret_cast<uint32_t *>(v.h + 24); }
 __device__ __forceinline__ uint32_t *q_tail(const view &v) { return reinterpret_cast<uint32_t *>(v.h + 28); }
@@ -188,6 +235,10 @@ __device__ __forceinline__ int sbid_by_size(uint32_t log2, uint64_t size)
 }
 
 /* ---- indexed queue (clIndexedQueue.cl:119-209) ---- */
+/* Enqueue is lock-free (a failed CAS means another operation went
+ * through), so it has no deadline: a superblock is never dropped from a
+ * consistent queue.  Only a corrupt queue (a cycle) can keep it looping;
+ * then the watchdog ends the call and the loss is counted. */
 __device__ inline void q_enqueue(const view &v, char *item)
 {
 	uint32_t tag = q_tag(ld(sb_next(item)));
@@ -195,9 +246,12 @@ __device__ inline void q_enqueue(const view &v, char *item)
 	uint32_t idx = ptr2idx(v, item);
 	uint32_t tailw;
 	waiter wt;
+	uint32_t bo = 1;
 	for (;;) {
-		if (wt.expired())
+		if (wt.expired(v, false)) {
+			count(v, T_ENQUEUE);
 			return;
+		}
 		tailw = ld(q_tail(v));
 		char *tail = idx2ptr(v, tailw);
 		uint32_t nextw = ld(sb_next(tail));
@@ -206,6 +260,7 @@ __device__ inline void q_enqueue(const view &v, char *item)
 		if (q_idx(nextw) == 0) {
 			if (cas(sb_next(tail), nextw, q_ptr(idx, q_tag(nextw))) == nextw)
 				break;
+			backoff(bo);
 		} else {
 			cas(q_tail(v), tailw, q_ptr(nextw, q_tag(tailw)));
 		}
@@ -213,12 +268,17 @@ __device__ inline void q_enqueue(const view &v, char *item)
 	cas(q_tail(v), tailw, q_ptr(idx, q_tag(tailw)));
 }
 
+/* Dequeue; NULL when the queue is empty, or when the wait for a consistent
+ * snapshot ends (watchdog or deadline; counted): nothing was taken then. */
 __device__ inline char *q_dequeue(const view &v)
 {
 	waiter wt;
+	uint32_t bo = 1;
 	for (;;) {
-		if (wt.expired())
+		if (wt.expired(v, true)) {
+			count(v, T_DEQUEUE);
 			return nullptr;
+		}
 		uint32_t headw = ld(q_head(v));
 		char *head = idx2ptr(v, headw);
 		uint32_t tailw = ld(q_tail(v));
@@ -231,20 +291,50 @@ __device__ inline char *q_dequeue(const view &v)
 			cas(q_tail(v), tailw, q_ptr(nextw, q_tag(tailw)));
 		} else if (cas(q_head(v), headw, q_ptr(nextw, q_tag(headw))) == headw) {
 			return head;
+		} else {
+			backoff(bo);
 		}
 	}
 }
 
+/* The queue holds no superblock to pop: the dummy node is both head and
+ * tail and has no successor (q_dequeue's empty case, by plain loads). */
+__device__ __forceinline__ bool q_empty(const view &v)
+{
+	uint32_t headw = ld(q_head(v));
+	uint32_t tailw = ld(q_tail(v));
+	if (q_idx(headw) != q_idx(tailw))
+		return false;
+	return q_idx(ld(sb_next(idx2ptr(v, headw)))) == 0;
+}
+
 /* ---- reserve + claim (kma.cl:150-273) ---- */
+/* A malloc that finds its class with no superblock attached and the free
+ * queue empty returns NULL after plain loads: the reference takes the class
+ * head's POISON lock for that (CAS 0 -> POISON, empty dequeue, release),
+ * which leaves the same image but serializes every failing malloc of an
+ * exhausted heap on one word. */
 __device__ inline char *reserve(const view &v, int block, uint32_t *slot)
 {
 	uint64_t *hp = cls_head(v, block);
+	if (past_deadline(v)) {
+		count(v, T_DLHIT);
+		return nullptr;
+	}
 	waiter wt;
 	uint32_t bo = 1;
 	for (;;) {
-		if (wt.expired())
+		int e = wt.expired(v, true);
+		if (e) {
+			count(v, e == 2 ? T_DLHIT : T_RESERVE);
 			return nullptr;
-		uint64_t cur = cas(hp, (uint64_t)0, POISON);
+		}
+		uint64_t cur = ld(hp);
+		if (cur == 0) {
+			if (q_empty(v))
+				return nullptr;
+			cur = cas(hp, (uint64_t)0, POISON);
+		}
 		if (cur == 0) {
 			char *sb = q_dequeue(v);
 			if (!sb) {
@@ -274,8 +364,10 @@ __device__ inline char *reserve(const view &v, int block, uint32_t *slot)
 				continue;
 			}
 			fr--;
-			if (cas(sb_state(sb), so, (so & 0xffff0000u) | fr) != so)
+			if (cas(sb_state(sb), so, (so & 0xffff0000u) | fr) != so) {
+				backoff(bo);
 				continue;
+			}
 			if (fr == 0)
 				xchg(hp, (uint64_t)0);
 			*slot = fr;
@@ -285,13 +377,40 @@ __device__ inline char *reserve(const view &v, int block, uint32_t *slot)
 	}
 }
 
+/* One slot of `sb` back into its free count (kma.cl:340-371): when every
+ * slot is free the superblock is reset, unlinked from its class head and
+ * enqueued.  Lock-free: a failed CAS means another count update went in. */
+__device__ inline void release_slot(const view &v, char *sb)
+{
+	uint32_t so, ns, bo = 1;
+	bool enq;
+	for (;;) {
+		so = ld(sb_state(sb));
+		uint32_t fr = (so & 0xffffu) + 1u, slots = so >> 16;
+		enq = fr == slots;
+		ns = (enq ? 0u : fr) | (slots << 16);
+		if (cas(sb_state(sb), so, ns) == so)
+			break;
+		backoff(bo);
+	}
+	if (enq) {
+		int c = sbid_by_size(v.log2, ld(sb_size(sb)));
+		cas(cls_head(v, c), (uint64_t)(uintptr_t)sb, (uint64_t)0);
+		q_enqueue(v, sb);
+	}
+}
+
+/* Claim the first free bit from `slot` upward, wrapping to 0.  The
+ * reservation guarantees a free bit in a consistent heap; if none turns up
+ * before the watchdog, the reserved slot is returned and the claim counted
+ * (NULL), so the superblock can still empty and go back to the queue. */
 __device__ inline char *claim(const view &v, char *sb, uint32_t slot)
 {
 	uint32_t slots = ld(sb_state(sb)) >> 16;
 	uint32_t bsize = ld(sb_size(sb));
 	uint32_t start = slot;
 	waiter wt;
-	while (!wt.expired()) {
+	do {
 		for (uint32_t b = slot; b < slots; b++) {
 			uint32_t *w = sb_word(v, sb, b);
 			uint32_t bit = 1u << (b & 31u);
@@ -304,7 +423,9 @@ __device__ inline char *claim(const view &v, char *sb, uint32_t slot)
 			}
 		}
 		slot = 0;
-	}
+	} while (!wt.expired(v, false));
+	count(v, T_CLAIM);
+	release_slot(v, sb);
 	return nullptr;
 }
 
@@ -343,19 +464,7 @@ __device__ inline void kma_free(clheap *heap, void *ptr)
 	/* old word into the block first, then release the bit */
 	*reinterpret_cast<volatile uint32_t *>(p) = ld(w);
 	__hip_atomic_fetch_and(w, ~(1u << (b & 31u)), __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-	uint32_t so, ns;
-	bool enq;
-	do {
-		so = ld(sb_state(sb));
-		uint32_t fr = (so & 0xffffu) + 1u, slots = so >> 16;
-		enq = fr == slots;
-		ns = (enq ? 0u : fr) | (slots << 16);
-	} while (cas(sb_state(sb), so, ns) != so);
-	if (enq) {
-		int c = sbid_by_size(v.log2, size);
-		cas(cls_head(v, c), (uint64_t)(uintptr_t)sb, (uint64_t)0);
-		q_enqueue(v, sb);
-	}
+	release_slot(v, sb);
 }
 
 #endif /* KMA_REF_DEVICE_H */

@@ -38,7 +38,10 @@ typedef struct kma_workload {
 	uint32_t fixed;   /* size for KMA_WL_FIXED */
 	uint64_t seed;
 	uint32_t warmup;  /* untimed rounds run first */
-	uint32_t reps;    /* reserved, 0 */
+	uint32_t deadline_ms; /* reference layout: every allocator wait that may
+				 fail gives up (NULL, counted in deadline_hits)
+				 once this many ms of the timed region have passed;
+				 0: no deadline, only the 4 s per-call watchdogs */
 	uint32_t variant; /* KMA_VAR_* bits */
 } kma_workload;
 
@@ -60,7 +63,11 @@ typedef struct kma_run_stats {
 	uint64_t double_frees;  /* heap counter (native layout) */
 	uint64_t bad_frees;     /* heap counter (native layout) */
 	uint64_t pool_empty;    /* refills that found no free superblock */
-	uint64_t waits[4];      /* watchdog expiries: reserve, claim, group claim, pool */
+	uint64_t waits[4];      /* native layout: watchdog expiries of reserve, claim,
+				   group claim, pool.  Reference layout: give-ups of
+				   reserve (watchdog: NULL), claim (slot returned: NULL),
+				   dequeue (refill saw an empty queue) and enqueue
+				   (corrupt queue: superblock lost) */
 	uint64_t remote_frees;  /* frees issued from a non-home XCD */
 	uint64_t migrated;      /* superblocks moved from an XCD's dirty shards to
 				   its clean shards inside a kernel */
@@ -69,6 +76,8 @@ typedef struct kma_run_stats {
 	double lat_max_ns;
 	double phase_ns[3];     /* KMA_VAR_PHASES: malloc / hold / free time per
 				   wavefront-round */
+	uint64_t deadline_hits; /* reference layout: mallocs that returned NULL
+				   because deadline_ms had passed */
 } kma_run_stats;
 
 /* Run a per-work-item workload on the heap. */
@@ -350,7 +359,7 @@ int kma_graph_build(kma_heap *heap, struct kma_pma *pma, const uint32_t *edges, 
 		    uint32_t nwg, int teardown, kma_graph_stats *out);
 
 /* ------------------------------------------------------------------ */
-/* Heap walker / orphan detector (native layout)                        */
+/* Heap walker / orphan detector                                        */
 /* ------------------------------------------------------------------ */
 /* The reference has a commented-out heap walker (kma.cl:528-548) and
  * prints heap dumps from its test bench (tb_kma.c:304-313); this is the
@@ -365,7 +374,20 @@ int kma_graph_build(kma_heap *heap, struct kma_pma *pma, const uint32_t *edges, 
  * Shard counts must equal the chain lengths, a non-empty shard's mask bit
  * must be set, the page bitmap must be a sequence of runs whose first pages
  * record their lengths, and the window summaries of both levels must be
- * exact.  Only valid while no kernel uses the heap. */
+ * exact.  Only valid while no kernel uses the heap.
+ *
+ * Reference layout (kma.h:71-85): the free queue is walked from its head
+ * the way the reference's queue validator walks it
+ * (tb_clIndexedQueue.c:131-168): it must end within S nodes (no cycle), at
+ * the node the tail names, and every queued superblock must be free (no
+ * bit set, free count 0, not a class head).  A class head must be empty or
+ * name a superblock of its class (POISON means a refill was cut off).
+ * Every superblock out of the queue and in use must count exactly its free
+ * slots (slots - set bits), have no bit past its slots and carry a block
+ * size of the table; one with no live block that is neither queued nor a
+ * class head is an orphan (the reference's commented-out check,
+ * kma.cl:528-548).  The last superblock, which the reference's init never
+ * queues (kma.cl:42-51), is counted in sb_unqueued, not as a problem. */
 typedef struct kma_heap_report {
 	uint64_t problems;     /* violations found (0: consistent) */
 	uint64_t sb_total;     /* superblocks */
@@ -379,13 +401,16 @@ typedef struct kma_heap_report {
 	uint64_t page_runs;    /* live page runs */
 	uint64_t pages_used;
 	uint64_t pages_total;
-	uint64_t counters[9];  /* the heap's event counters, KMA_STAT_* order */
+	uint64_t sb_unqueued;  /* reference layout: never queued, never used */
+	uint64_t counters[9];  /* native: the heap's event counters, KMA_STAT_* order;
+				  reference: the stats trailer's words 0..7 */
 	char first[1024];      /* the first problems, one per line */
 } kma_heap_report;
 
-/* Check a metadata image: bytes [0, off_data) of a native heap, as
- * kma_snapshot(h, 0, ...) returns them.  0 with *out filled (problems
- * counted there), -1 if the image is not a native heap's. */
+/* Check a metadata image: bytes [0, off_data) of a native heap, or the
+ * whole image of a reference-layout heap, as kma_snapshot(h, 0, ...)
+ * returns them.  0 with *out filled (problems counted there), -1 if the
+ * image is neither heap's. */
 int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out);
 
 /* Snapshot h's metadata and check it (no kernel may be using h). */

@@ -70,7 +70,7 @@ class Workload(ctypes.Structure):
         ("fixed", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
         ("warmup", ctypes.c_uint32),
-        ("reps", ctypes.c_uint32),
+        ("deadline_ms", ctypes.c_uint32),
         ("variant", ctypes.c_uint32),
     ]
 
@@ -94,6 +94,7 @@ class RunStats(ctypes.Structure):
         ("lat_p99_ns", ctypes.c_double),
         ("lat_max_ns", ctypes.c_double),
         ("phase_ns", ctypes.c_double * 3),
+        ("deadline_hits", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -190,11 +191,11 @@ class GraphStats(ctypes.Structure):
 class HeapReport(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in (
         "problems", "sb_total", "sb_free", "sb_in_spares", "sb_heads", "sb_detached", "orphans", "live_blocks",
-        "live_bytes", "page_runs", "pages_used", "pages_total")] + [
+        "live_bytes", "page_runs", "pages_used", "pages_total", "sb_unqueued")] + [
         ("counters", ctypes.c_uint64 * 9), ("first", ctypes.c_char * 1024)]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_[:12]}
+        d = {k: getattr(self, k) for k, _ in self._fields_[:13]}
         d["counters"] = list(self.counters)
         d["first"] = self.first.decode(errors="replace").splitlines()
         return d
@@ -464,8 +465,8 @@ class Heap:
         return ms.value
 
     def run(self, mode, nwi, rounds, form=FORM_PLAIN, block=64, fixed=64, seed=1,
-            round0=0, warmup=0, variant=0):
-        w = Workload(mode, form, nwi, block, rounds, round0, fixed, seed, warmup, 0, variant)
+            round0=0, warmup=0, variant=0, deadline_ms=0):
+        w = Workload(mode, form, nwi, block, rounds, round0, fixed, seed, warmup, deadline_ms, variant)
         st = RunStats()
         _check(lib().kma_run_workload(self._h, ctypes.byref(w), ctypes.byref(st)), "kma_run_workload")
         return st.as_dict()

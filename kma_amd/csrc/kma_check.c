@@ -1,12 +1,16 @@
 /*
- * kma_check.c - heap walker / orphan detector for the native layout (the
- * reference's commented-out heap walker, kma.cl:528-548, and its heap
- * dumps, tb_kma.c:304-313).  Works on a metadata image (kma_snapshot of
- * bytes [0, off_data)) taken while no kernel uses the heap: every
- * superblock must be in exactly one place (a pool shard, a context spare
- * line, a context head, or in use with live blocks), its count must equal
- * its live bits, and the page bitmap must match the run lengths and their
- * window summaries.
+ * kma_check.c - heap walker / orphan detector (the reference's
+ * commented-out heap walker, kma.cl:528-548, and its heap dumps,
+ * tb_kma.c:304-313) for both layouts.  Works on an image taken while no
+ * kernel uses the heap.  Native layout (kma_snapshot of bytes
+ * [0, off_data)): every superblock must be in exactly one place (a pool
+ * shard, a context spare line, a context head, or in use with live
+ * blocks), its count must equal its live bits, and the page bitmap must
+ * match the run lengths and their window summaries.  Reference layout (the
+ * whole image): the free queue is walked as the reference's validator does
+ * (tb_clIndexedQueue.c:131-168) and every superblock's state word, block
+ * size and bitmap must agree with where it is (queued, a class head, or
+ * detached with live blocks).
  */
 #define __HIP_PLATFORM_AMD__ 1
 #include <hip/hip_runtime_api.h>
@@ -81,16 +85,210 @@ static uint64_t page_windows(uint64_t w, uint32_t l, uint64_t valid)
 	return z & pat;
 }
 
+/* ---------------- reference layout ---------------- */
+/* block size of class c and slots per superblock (kma.cl:55-114; the same
+ * arithmetic as the device code in kma_ref_device.h) */
+static uint32_t ref_size_by_sbid(uint32_t log2, int c)
+{
+	if (c < 0 || c > (int)log2 - 1)
+		return 0;
+	int e = c + 2;
+	if (e <= (int)(log2 >> 1))
+		return 1u << e;
+	uint32_t items = (1u << log2) >> e;
+	uint32_t ab = (items >> 3) + ((items & 7u) ? 1u : 0u);
+	ab = (ab + 3u) & ~3u;
+	return (((1u << log2) - 12u - ab) / items) & ~3u;
+}
+
+static uint32_t ref_slots_by_size(uint32_t log2, uint32_t size)
+{
+	uint32_t space = (1u << log2) - 12u;
+	uint32_t s = space / size;
+	if (s & 0x1fu)
+		s += 32u;
+	s = (s >> 3) & ~3u;
+	return (space - s) / size;
+}
+
+/* 1 if the header is a reference-layout heap's (kma.h:81-85): `bytes`
+ * = S superblocks + header, free.stride = log2 of the superblock size,
+ * free.base != 0. */
+static int ref_header(const uint8_t *img, uint64_t bytes, uint32_t *log2_out, uint64_t *S_out)
+{
+	if (bytes < 32)
+		return 0;
+	uint64_t hb = rd64(img, 0), log2 = rd64(img, 16);
+	if (log2 < 8 || log2 > 16)
+		return 0;
+	uint64_t hdr = 32 + 8 * (log2 - 1);
+	if (hb < hdr || hb > bytes || rd64(img, 8) == 0)
+		return 0;
+	uint64_t S = (hb - hdr) >> log2;
+	if (S < 2 || S > 65536 || hb != (S << log2) + hdr)
+		return 0;
+	*log2_out = (uint32_t)log2;
+	*S_out = S;
+	return 1;
+}
+
+static int ref_image_check(const uint8_t *img, uint32_t log2, uint64_t S, kma_heap_report *out)
+{
+	rep p = {out, 0};
+	const uint32_t buckets = log2 - 1u;
+	const uint64_t hdr = 32 + 8ull * buckets, sbsz = 1ull << log2, sb0 = rd64(img, 8);
+	uint8_t *inq = (uint8_t *)calloc(S, 1);
+	uint8_t *head_of = (uint8_t *)calloc(S, 1); /* class + 1 */
+	if (!inq || !head_of) {
+		free(inq);
+		free(head_of);
+		set_err("host allocation failed");
+		return -1;
+	}
+	/* the free queue, from its head: nodes are 1-based indexes in bits
+	 * 1..16 of the links (clIndexedQueue.cl:89-91) */
+	uint32_t headw = rd32(img, 24), tailw = rd32(img, 28);
+	uint32_t hi = (headw & 0x1fffeu) >> 1, ti = (tailw & 0x1fffeu) >> 1;
+	if ((headw | tailw) & 1u)
+		problem(&p, "queue head %#x / tail %#x carries the poison bit", headw, tailw);
+	if (ti == 0 || ti > S)
+		problem(&p, "queue tail names node %u of %llu", ti, (unsigned long long)S);
+	if (hi == 0 || hi > S) {
+		problem(&p, "queue head names node %u of %llu", hi, (unsigned long long)S);
+	} else {
+		uint64_t n = 0;
+		for (uint32_t i = hi;;) {
+			if (inq[i - 1u]) {
+				problem(&p, "queue links back to superblock %u (a cycle)", i - 1u);
+				break;
+			}
+			inq[i - 1u] = 1;
+			n++;
+			uint32_t nw = rd32(img, hdr + (uint64_t)(i - 1u) * sbsz);
+			if (nw & 1u)
+				problem(&p, "queued superblock %u has a poisoned link %#x", i - 1u, nw);
+			uint32_t ni = (nw & 0x1fffeu) >> 1;
+			if (ni == 0) {
+				if (i != ti)
+					problem(&p, "queue ends at superblock %u but its tail names %u", i - 1u, ti - 1u);
+				break;
+			}
+			if (ni > S) {
+				problem(&p, "queued superblock %u links to node %u, past the heap", i - 1u, ni);
+				break;
+			}
+			i = ni;
+		}
+		out->sb_free = n;
+	}
+	/* class heads */
+	for (uint32_t c = 0; c < buckets; c++) {
+		uint64_t v = rd64(img, 32 + 8ull * c);
+		if (v == 0)
+			continue;
+		if (v == 1) {
+			problem(&p, "class %u head is POISON (a refill did not finish)", c);
+			continue;
+		}
+		if (v < sb0 || (v - sb0) % sbsz || (v - sb0) / sbsz >= S) {
+			problem(&p, "class %u head %#llx is not a superblock", c, (unsigned long long)v);
+			continue;
+		}
+		uint64_t s = (v - sb0) / sbsz;
+		if (head_of[s])
+			problem(&p, "superblock %llu heads classes %u and %u", (unsigned long long)s, head_of[s] - 1u, c);
+		head_of[s] = (uint8_t)(c + 1u);
+		out->sb_heads++;
+		uint32_t size = rd32(img, hdr + s * sbsz + 8);
+		if (size != ref_size_by_sbid(log2, (int)c))
+			problem(&p, "class %u head: superblock %llu holds %u-byte blocks", c, (unsigned long long)s, size);
+	}
+	/* superblocks */
+	for (uint64_t s = 0; s < S; s++) {
+		const uint64_t so = hdr + s * sbsz;
+		uint32_t state = rd32(img, so + 4), size = rd32(img, so + 8);
+		uint32_t slots = state >> 16, fr = state & 0xffffu;
+		uint64_t live = 0, beyond = 0;
+		uint32_t words = (slots + 31u) / 32u;
+		if (words * 4ull > sbsz - 12)
+			words = 0; /* a corrupt slot count: reported below */
+		for (uint32_t k = 0; k < words; k++) {
+			uint32_t b = rd32(img, so + sbsz - 4ull * (k + 1u));
+			live += (uint64_t)__builtin_popcount(b);
+			for (uint32_t j = 0; j < 32; j++)
+				if (((b >> j) & 1u) && k * 32u + j >= slots)
+					beyond++;
+		}
+		if (inq[s]) {
+			if (head_of[s])
+				problem(&p, "queued superblock %llu is also the head of class %u", (unsigned long long)s,
+					head_of[s] - 1u);
+			if (fr || live)
+				problem(&p, "queued superblock %llu: free count %u, %llu bits set", (unsigned long long)s, fr,
+					(unsigned long long)live);
+			continue;
+		}
+		if (s == S - 1 && state == 0 && size == 0 && !head_of[s] && rd32(img, so) == 0) {
+			out->sb_unqueued++; /* never queued by the init (kma.cl:42-51) */
+			continue;
+		}
+		int c = -1;
+		for (uint32_t k = 0; k < buckets && c < 0; k++)
+			if (size && ref_size_by_sbid(log2, (int)k) == size)
+				c = (int)k;
+		if (c < 0) {
+			problem(&p, "superblock %llu is in no queue and holds %u-byte blocks, no class's size",
+				(unsigned long long)s, size);
+			continue;
+		}
+		uint32_t want = ref_slots_by_size(log2, size);
+		if (slots != want) {
+			problem(&p, "superblock %llu of class %d: %u slots, the class has %u", (unsigned long long)s, c,
+				slots, want);
+			continue;
+		}
+		if (beyond)
+			problem(&p, "superblock %llu has %llu bits past its %u slots", (unsigned long long)s,
+				(unsigned long long)beyond, slots);
+		if ((uint64_t)fr + live != slots)
+			problem(&p, "superblock %llu counts %u free slots, %llu of %u bits are set", (unsigned long long)s, fr,
+				(unsigned long long)live, slots);
+		if (!head_of[s]) {
+			if (live == 0) {
+				out->orphans++;
+				problem(&p, "superblock %llu of class %d has no live block, is no class head and is not "
+					    "queued (orphan)",
+					(unsigned long long)s, c);
+			} else {
+				out->sb_detached++;
+			}
+		}
+		out->live_blocks += live;
+		out->live_bytes += live * size;
+	}
+	out->sb_total = S;
+	free(inq);
+	free(head_of);
+	return 0;
+}
+
 int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 {
-	if (!image || !out || bytes < sizeof(nat_hdr)) {
+	if (!image || !out || bytes < 32) {
 		set_err("bad arguments");
 		return -1;
 	}
 	memset(out, 0, sizeof(*out));
 	const uint8_t *img = (const uint8_t *)image;
 	nat_hdr h;
-	memcpy(&h, img, sizeof(h));
+	if (bytes < sizeof(nat_hdr) || (memcpy(&h, img, sizeof(h)), h.magic != NAT_MAGIC)) {
+		uint32_t log2;
+		uint64_t S;
+		if (ref_header(img, bytes, &log2, &S))
+			return ref_image_check(img, log2, S, out);
+		set_err("neither a native nor a reference-layout heap image (or truncated)");
+		return -1;
+	}
 	if (h.magic != NAT_MAGIC || h.off_data > bytes || h.nctx == 0 || h.npool < 8 || h.nclasses == 0 ||
 	    h.nclasses > 32 || h.spare_slots > 16) {
 		set_err("not a native heap image (or truncated before its data offset)");
@@ -292,9 +490,25 @@ int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 
 int kma_heap_check(kma_heap *h, kma_heap_report *out)
 {
-	if (!h || !out || h->cfg.layout != KMA_LAYOUT_NATIVE) {
-		set_err("bad arguments (a native heap)");
+	if (!h || !out) {
+		set_err("bad arguments");
 		return -1;
+	}
+	if (h->cfg.layout == KMA_LAYOUT_REFERENCE) {
+		uint8_t *img = (uint8_t *)malloc(h->bytes);
+		uint64_t tw[REF_T_WORDS];
+		if (!img) {
+			set_err("host allocation failed");
+			return -1;
+		}
+		int rc = -1;
+		if (kma_snapshot(h, 0, img, h->bytes) == 0 && kma_i_read_ref_trailer(h, tw) == 0) {
+			rc = kma_image_check(img, h->bytes, out);
+			for (uint32_t k = 0; rc == 0 && k < REF_T_WORDS; k++)
+				out->counters[k] = tw[k];
+		}
+		free(img);
+		return rc;
 	}
 	uint64_t n = h->nh.off_data;
 	uint8_t *img = (uint8_t *)malloc(n);

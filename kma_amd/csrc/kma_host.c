@@ -65,7 +65,8 @@ static kma_heap *create_reference(const kma_config *cfg)
 	h->cfg = *cfg;
 	uint32_t hdr = 32u + 8u * (log2 - 1u);
 	h->bytes = (cfg->sblocks << log2) + hdr;
-	if (hipMalloc((void **)&h->dev, h->bytes) != hipSuccess) {
+	/* the image, then the stats trailer the device code finds after it */
+	if (hipMalloc((void **)&h->dev, REF_TRAILER_OFF(h->bytes) + REF_TRAILER_BYTES) != hipSuccess) {
 		set_err("KMA: Could not allocate heap on-device (%llu bytes)", (unsigned long long)h->bytes);
 		free(h);
 		return NULL;
@@ -130,8 +131,9 @@ int kma_reset(kma_heap *h)
 		return -1;
 	if (h->cfg.layout == KMA_LAYOUT_REFERENCE) {
 		/* The reference does not clear its buffer (kma.c:38); the heap
-		 * image this library defines starts from zero. */
-		HIPCHK(hipMemset(h->dev, 0, h->bytes), -1);
+		 * image this library defines starts from zero (the stats trailer
+		 * and its deadline too). */
+		HIPCHK(hipMemset(h->dev, 0, REF_TRAILER_OFF(h->bytes) + REF_TRAILER_BYTES), -1);
 		LCHK(kmad_ref_init(h->dev, h->cfg.sb_log2, (uint32_t)h->cfg.sblocks), -1);
 		return 0;
 	}
@@ -279,6 +281,16 @@ int kma_i_read_native_stats(kma_heap *h, uint64_t *st)
 	return 0;
 }
 
+int kma_i_read_ref_trailer(kma_heap *h, uint64_t *tw)
+{
+	if (h->cfg.layout != KMA_LAYOUT_REFERENCE) {
+		memset(tw, 0, REF_T_WORDS * 8);
+		return 0;
+	}
+	HIPCHK(hipMemcpy(tw, h->dev + REF_TRAILER_OFF(h->bytes), REF_T_WORDS * 8, hipMemcpyDeviceToHost), -1);
+	return 0;
+}
+
 static int cmp_u32(const void *a, const void *b)
 {
 	uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
@@ -329,7 +341,11 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	hipGraph_t graph = NULL;
 	hipGraphExec_t gexec = NULL;
 	int rc = -1;
-	uint64_t st0[NAT_NSTATS], st1[NAT_NSTATS];
+	uint64_t st0[NAT_NSTATS], st1[NAT_NSTATS], tw0[REF_T_WORDS], tw1[REF_T_WORDS];
+	int ref = h->cfg.layout == KMA_LAYOUT_REFERENCE;
+	uint64_t budget = 0;
+	if (ref && wl->deadline_ms)
+		budget = (uint64_t)((double)wl->deadline_ms * 1e6 / kma_i_wall_clock_ns());
 	unsigned long long hs[8];
 	uint64_t nlat = (wl->variant & KMA_VAR_LATENCY) ? wl->nwi * wl->rounds : 0;
 	uint32_t untimed = wl->variant & ~(KMA_VAR_LATENCY | KMA_VAR_PHASES);
@@ -351,7 +367,7 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		}
 	}
 	if (hipStreamSynchronize(s) != hipSuccess || hipMemset(d_stats, 0, sizeof(hs)) != hipSuccess ||
-	    kma_i_read_native_stats(h, st0) != 0) {
+	    kma_i_read_native_stats(h, st0) != 0 || kma_i_read_ref_trailer(h, tw0) != 0) {
 		set_err("warm-up failed");
 		rc = -1;
 		goto done;
@@ -371,6 +387,12 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 			rc = -1;
 			goto done;
 		}
+	}
+	/* reference layout: the deadline runs from here over every launch of
+	 * the timed region (every wait gives up once it has passed) */
+	if (budget && (rc = kmad_ref_arm(h->dev, budget, s)) != 0) {
+		set_err("deadline: launch failed: %d", rc);
+		goto done;
 	}
 	hipEventRecord(e0, s);
 	if (wl->form == KMA_FORM_PLAIN) {
@@ -400,7 +422,11 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 		out->launches = 1;
 	}
 	hipEventRecord(e1, s);
-	if (hipEventSynchronize(e1) != hipSuccess) {
+	if (budget && (rc = kmad_ref_arm(h->dev, 0, s)) != 0) {
+		set_err("deadline: launch failed: %d", rc);
+		goto done;
+	}
+	if (hipEventSynchronize(e1) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
 		set_err("workload kernels failed: %s", hipGetErrorString(hipGetLastError()));
 		rc = -1;
 		goto done;
@@ -409,7 +435,8 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	hipEventElapsedTime(&ms, e0, e1);
 	out->gpu_ms = ms;
 	out->kernel_ms_avg = out->launches ? ms / (double)out->launches : 0;
-	if (hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess || kma_i_read_native_stats(h, st1) != 0) {
+	if (hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess || kma_i_read_native_stats(h, st1) != 0 ||
+	    kma_i_read_ref_trailer(h, tw1) != 0) {
 		set_err("stats download failed");
 		rc = -1;
 		goto done;
@@ -423,6 +450,15 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	out->pool_empty = st1[2] - st0[2];
 	for (int k = 0; k < 4; k++)
 		out->waits[k] = st1[3 + k] - st0[3 + k];
+	if (ref) {
+		/* trailer words 1..5: reserve watchdog, deadline, claim, dequeue,
+		 * enqueue */
+		out->waits[0] = tw1[1] - tw0[1];
+		out->waits[1] = tw1[3] - tw0[3];
+		out->waits[2] = tw1[4] - tw0[4];
+		out->waits[3] = tw1[5] - tw0[5];
+		out->deadline_hits = tw1[2] - tw0[2];
+	}
 	out->remote_frees = st1[7] - st0[7];
 	out->migrated = st1[8] - st0[8];
 	double tick = kma_i_wall_clock_ns();

@@ -15,6 +15,12 @@
 #define NAT_STAT_DOUBLE_FREE 0
 #define NAT_STAT_BAD_FREE 1
 
+/* Reference layout: stats trailer after the image (kma_ref_device.h,
+ * kma_ref::T_* word indexes): at align64(image bytes), 256 bytes. */
+#define REF_TRAILER_BYTES 256u
+#define REF_TRAILER_OFF(bytes) (((bytes) + 63u) & ~63ull)
+#define REF_T_WORDS 8u
+
 struct kma_heap {
 	kma_config cfg;
 	char *dev;        /* device buffer == header address */
@@ -28,6 +34,10 @@ __attribute__((visibility("hidden"), format(printf, 1, 2))) void kma_i_set_err(c
 
 /* The native heap's 8 counters (zeros for the reference layout). */
 __attribute__((visibility("hidden"))) int kma_i_read_native_stats(kma_heap *h, uint64_t *st);
+
+/* The reference heap's trailer words (REF_T_WORDS; zeros for the native
+ * layout). */
+__attribute__((visibility("hidden"))) int kma_i_read_ref_trailer(kma_heap *h, uint64_t *tw);
 
 /* Nanoseconds per tick of the device's constant wall clock. */
 __attribute__((visibility("hidden"))) double kma_i_wall_clock_ns(void);

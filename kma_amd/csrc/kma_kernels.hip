@@ -80,6 +80,22 @@ extern "C" int kmad_ref_init(void *heap, uint32_t log2, uint32_t sblocks)
 	return 0;
 }
 
+__global__ void k_ref_arm(clheap *heap, uint64_t budget)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		kma_ref::view v = kma_ref::make_view(heap);
+		uint64_t dl = budget ? wall_clock64() + budget : 0;
+		__hip_atomic_store(v.trl + kma_ref::T_DEADLINE, dl, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+	}
+}
+
+extern "C" int kmad_ref_arm(void *heap, uint64_t budget_ticks, void *stream)
+{
+	hipLaunchKernelGGL(k_ref_arm, dim3(1), dim3(64), 0, (hipStream_t)stream, (clheap *)heap, budget_ticks);
+	KMA_CHECK(hipGetLastError());
+	return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* native layout: metadata init                                         */
 /* ------------------------------------------------------------------ */

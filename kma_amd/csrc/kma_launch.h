@@ -15,6 +15,10 @@ extern "C" {
 /* Reference layout: parallel clheap_init producing the serial init's image. */
 int kmad_ref_init(void *heap, uint32_t log2, uint32_t sblocks);
 
+/* Reference layout: set the stats trailer's deadline to the device wall
+ * clock + budget_ticks when the kernel runs on `stream` (0: no deadline). */
+int kmad_ref_arm(void *heap, uint64_t budget_ticks, void *stream);
+
 /* Native layout: metadata init (header already copied to device). */
 int kmad_nat_init(void *heap, uint64_t nsb, uint32_t npool, uint64_t nheads,
 		  uint64_t npbwords);

@@ -30,6 +30,7 @@ static uint64_t sblocks = 0;           /* -H: superblocks (0: 127, or 65536 nati
 static uint32_t nctx = 0;              /* -C: native allocation contexts (0: 4096) */
 static unsigned block = 256;           /* -b: work-group size */
 static unsigned samples = 1;           /* -s: timed launches per point */
+static unsigned deadline_ms = 5000;    /* -D: reference-layout deadline per launch */
 static const char *only = NULL;        /* -k: one kernel */
 
 #define TB_MAX_THREADS (1ull << 26)
@@ -97,6 +98,10 @@ static int kma_opts(int i, int argc, char **argv)
 		samples = (unsigned)v;
 		return 1;
 	}
+	if (strncmp(a, "-D", 2) == 0 && parse_uint(argv[i + 1], 0, 3600000, &v) == 0) {
+		deadline_ms = (unsigned)v;
+		return 1;
+	}
 	if (strncmp(a, "-k", 2) == 0) {
 		for (unsigned k = 0; k < sizeof(kernels) / sizeof(kernels[0]); k++)
 			if (strcmp(argv[i + 1], kernels[k].name) == 0) {
@@ -110,18 +115,22 @@ static int kma_opts(int i, int argc, char **argv)
 static void usage(void)
 {
 	printf("Usage: kma [options]\n");
-	printf("\t-i [step,]iters:Iteration count for kernel (default: 100)\n");
+	printf("Times the allocator's three test kernels, one launch per launch shape and\n");
+	printf("loop count.\n");
+	printf("  -i [step,]n  loop count per work-item (default 100); with a step, sweep the\n");
+	printf("               counts 2, step, 2*step, ... up to n\n");
 	tb_options_print();
-	printf("\t-N:\t\tNative heap layout (default: the reference layout, 4 KiB superblocks)\n");
-	printf("\t-H n:\t\tSuperblocks in the heap (default: 127; native: 65536 of 16 KiB)\n");
-	printf("\t-C n:\t\tNative allocation contexts, a power of two (default: 4096)\n");
-	printf("\t-b n:\t\tWork-group size, a multiple of 64 up to 256 (default: 256)\n");
-	printf("\t-s n:\t\tTimed launches per point (default: 1)\n");
-	printf("\t-k name:\tRun one kernel: kma_test_malloc, kma_test_malloc_lowvar or\n");
-	printf("\t\t\tkma_test_malloc_highvar (default: all three in that order)\n");
-	printf("\n");
-	printf("For -i, if step is defined, the program will run each thread config\n");
-	printf("with {2, step, 2*step..iters} iterations\n");
+	printf("  -N           use the native heap layout (default: the reference layout with\n");
+	printf("               4 KiB superblocks)\n");
+	printf("  -H n         superblocks per heap (default 127; native 65536 of 16 KiB)\n");
+	printf("  -C n         native allocation contexts, a power of two (default 4096)\n");
+	printf("  -b n         work-group size, a multiple of 64 up to 256 (default 256)\n");
+	printf("  -s n         timed launches per point (default 1)\n");
+	printf("  -D ms        reference layout: waits give up after ms per launch, the\n");
+	printf("               mallocs return NULL and the run line counts them (default\n");
+	printf("               5000; 0: only the 4 s per-call watchdogs)\n");
+	printf("  -k name      one kernel only: kma_test_malloc, kma_test_malloc_lowvar or\n");
+	printf("               kma_test_malloc_highvar (default: all three, in that order)\n");
 }
 
 /* Iteration counts of one thread configuration (tb_kma.c:250-296): iters
@@ -180,6 +189,7 @@ static int run_kernel(unsigned k)
 					wl.nwi = threads;
 					wl.block = block;
 					wl.rounds = (uint32_t)its;
+					wl.deadline_ms = native ? 0 : deadline_ms;
 					int ok = threads > 0 && threads <= TB_MAX_THREADS;
 					if (!ok || kma_run_workload(h, &wl, &st) != 0) {
 						printf("KMA_test: Could not execute %s on %" PRIu64 " threads: %s\n",
@@ -193,6 +203,11 @@ static int run_kernel(unsigned k)
 					break;
 				printf("%-5" PRIu64 " threads %-5" PRIu64 " iters: ", threads, its);
 				tb_print_times(us, samples);
+				if (!native && (st.deadline_hits || st.waits[0] || st.waits[1] || st.waits[2] || st.waits[3]))
+					printf("\tgave up: deadline %" PRIu64 " reserve %" PRIu64 " claim %" PRIu64 " dequeue %" PRIu64
+					       " enqueue %" PRIu64 " (nulls %" PRIu64 " of %" PRIu64 " mallocs)\n",
+					       st.deadline_hits, st.waits[0], st.waits[1], st.waits[2], st.waits[3], st.nulls,
+					       st.mallocs);
 				if (tb_opt.debug) {
 					printf("\tops %" PRIu64 " mallocs %" PRIu64 " nulls %" PRIu64 " bad %" PRIu64
 					       " double_frees %" PRIu64 " bad_frees %" PRIu64 " waits %" PRIu64 " %" PRIu64
@@ -200,7 +215,7 @@ static int run_kernel(unsigned k)
 					       st.ops, st.mallocs, st.nulls, st.bad, st.double_frees, st.bad_frees,
 					       st.waits[0], st.waits[1], st.waits[2], st.waits[3],
 					       st.gpu_ms > 0 ? (double)st.ops / (st.gpu_ms * 1e-3) : 0.0);
-					if (native) {
+					{
 						kma_heap_report r;
 						if (kma_heap_check(h, &r) == 0)
 							printf("\theap walk: %" PRIu64 " problems, %" PRIu64 " orphans, %" PRIu64

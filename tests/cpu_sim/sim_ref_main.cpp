@@ -42,8 +42,10 @@ int main(int argc, char **argv)
 	const int mode = atoi(argv[6]);
 	const uint32_t B = 256;
 	uint64_t bytes = rm_heap_bytes(log2, S);
-	uint8_t *img = (uint8_t *)aligned_alloc(4096, (bytes + 4095) & ~4095ull);
-	memset(img, 0, bytes);
+	/* the image, then the 256-byte stats trailer at the next 64-byte boundary */
+	uint64_t alloc = (((bytes + 63) & ~63ull) + 256 + 4095) & ~4095ull;
+	uint8_t *img = (uint8_t *)aligned_alloc(4096, alloc);
+	memset(img, 0, alloc);
 	rm_heap h;
 	if (rm_init(&h, log2, S, (uint64_t)(uintptr_t)img, img) != RM_OK) {
 		fprintf(stderr, "rm_init failed\n");
@@ -73,7 +75,9 @@ int main(int argc, char **argv)
 	for (auto &t : th)
 		t.join();
 	int64_t q = rm_queue_walk(&h);
-	printf("ops %llu nulls %llu bad %llu queue %lld\n", stats[0], stats[1], stats[2], (long long)q);
+	const uint64_t *trl = (const uint64_t *)(img + ((bytes + 63) & ~63ull));
+	printf("ops %llu nulls %llu bad %llu queue %lld giveups %llu\n", stats[0], stats[1], stats[2], (long long)q,
+	       (unsigned long long)(trl[1] + trl[2] + trl[3] + trl[4] + trl[5]));
 	rm_release(&h);
 	free(img);
 	return 0;

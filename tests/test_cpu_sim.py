@@ -83,7 +83,23 @@ def test_reference_layout_in_simulation(sim, args):
     vals = dict(zip(f[0::2], f[1::2]))
     assert int(vals["bad"]) == 0, out.stdout
     assert int(vals["queue"]) == args[1] - 1, out.stdout
+    assert int(vals["giveups"]) == 0, out.stdout
     nwi, rounds = args[3], args[4]
     extra = 0 if args[5] == 2 else 2 * rounds  # work-item 0's 4000-B block
     if int(vals["nulls"]) == 0:  # a NULL ends its work-item early
         assert int(vals["ops"]) == 2 * nwi * rounds + extra, out.stdout
+
+
+def test_reference_layout_bounded_waits(sim):
+    """Every bounded wait of the reference-layout allocator on one simulated
+    work-item with a 50 ms watchdog (tests/cpu_sim/sim_ref_giveup.cpp): a
+    passed deadline makes malloc return NULL at once and is counted; a claim
+    on a bitmap planted full gives up, returns its reserved slot to the
+    superblock's free count (state word unchanged) and is counted; an
+    exhausted heap returns NULL with no give-up and, after every free, the
+    oracle walks all S-1 superblocks in the free queue."""
+    out = subprocess.run([os.path.join(HERE, "_build", "kma_sim_ref_giveup")], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    for case in ("deadline", "claim", "exhaust"):
+        assert "%s ok" % case in out.stdout, out.stdout

@@ -68,7 +68,42 @@ def test_page_runs_counted():
         assert r["page_runs"] == 2 and r["pages_used"] == 65, r
 
 
-def test_reference_layout_rejected():
+@pytest.mark.parametrize("mode", [kma_amd.WL_REF_TEST, kma_amd.WL_LOWVAR, kma_amd.WL_HIGHVAR])
+def test_reference_layout_walk_after_bench_kernels(mode):
+    """The reference's bench kernels (kma.cl:424-526) on kma_create(127) at
+    32,768 work-items x 20 iterations in one launch (as tb_kma.c runs them),
+    then the reference-layout walk: the free queue ends at its tail, no
+    superblock is lost, orphaned or miscounted, and no give-up was counted.
+    Every block is free again except, in _lowvar and _highvar, work-item 0's
+    small block when its malloc(4000) (a 1-slot class, which needs a whole
+    superblock) got NULL: the kernels return without freeing it
+    (kma.cl:484-486, 516-518)."""
+    print("heap check: reference layout mode=%d" % mode, flush=True)
+    with kma_amd.Heap.create(127) as h:
+        r = h.check()
+        assert r["problems"] == 0 and r["sb_free"] == 126 and r["sb_unqueued"] == 1, r
+        st = h.run(mode, 32768, 20, block=256, form=kma_amd.FORM_FUSED)
+        assert st["bad"] == 0 and sum(st["waits"]) == 0 and st["deadline_hits"] == 0, st
+        r = h.check()
+        assert r["problems"] == 0, r["first"]
+        leaked = 0 if mode == kma_amd.WL_REF_TEST else 1
+        assert r["live_blocks"] <= leaked and r["orphans"] == 0, r
+        if r["live_blocks"]:
+            assert st["nulls"] > 0 and r["live_bytes"] <= 64, (r, st)
+        else:
+            assert r["sb_heads"] == 0 and r["sb_free"] == 126, r
+        assert r["sb_total"] == 127 and r["counters"][0] == 0, r  # deadline disarmed
+
+
+def test_reference_layout_walk_with_live_blocks():
+    """A serialized trace leaving blocks live in four classes: the walk
+    counts exactly the blocks the trace left and the superblocks it used."""
+    print("heap check: reference layout live blocks", flush=True)
     with kma_amd.Heap(kma_amd.LAYOUT_REFERENCE, 12, 64) as h:
-        with pytest.raises(kma_amd.KmaError):
-            h.check()
+        ops = [64] * 70 + [8] * 10 + [1020] * 9 + [2040] * 3 + [-1, -2, -71]
+        out = h.trace_serial(ops)
+        assert np.count_nonzero(out[:92]) == 92
+        r = h.check()
+        assert r["problems"] == 0, r["first"]
+        assert r["live_blocks"] == 92 - 3, r
+        assert r["live_bytes"] == 68 * 64 + 9 * 8 + 9 * 1020 + 3 * 2040, r

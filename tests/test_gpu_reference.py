@@ -113,6 +113,27 @@ def test_reference_bench_kernels(mode):
         assert refimg.ref_canonical_free(h.snapshot(), 12, 127) == []
 
 
+def test_reference_kernel_deadline_bounds_the_launch():
+    """kma_test_malloc (kma.cl:424-461) at 131,072 work-items x 100 iterations on
+    the 127-superblock heap: the reference protocol needs seconds here (every
+    malloc and free CASes one superblock's state word).  With a 300 ms
+    deadline the launch ends soon after it, the mallocs that gave up are
+    counted and returned NULL, nothing was lost or stranded (claim and
+    enqueue give-ups 0), and the heap returns to its canonical free form."""
+    with kma_amd.Heap.create(127) as h:
+        _say("ref bench kernel with deadline")
+        st = h.run(kma_amd.WL_REF_TEST, 131072, 100, block=256, deadline_ms=300)
+        assert st["deadline_hits"] > 0, st
+        assert st["nulls"] >= st["deadline_hits"] + st["waits"][0], st
+        assert st["waits"][1] == 0 and st["waits"][3] == 0, st
+        assert st["bad"] == 0
+        assert st["gpu_ms"] < 4000, st["gpu_ms"]
+        assert refimg.ref_canonical_free(h.snapshot(), 12, 127) == []
+        # disarmed afterwards: a small run completes without a give-up
+        st = h.run(kma_amd.WL_REF_TEST, 4096, 10, block=64)
+        assert st["nulls"] == 0 and st["deadline_hits"] == 0, st
+
+
 def test_full_occupancy_workload_reference_layout():
     """Mixed classes on the reference layout, concurrently (1 GiB heap at
     16 KiB superblocks, 16,384 work-items, 2 rounds)."""
