@@ -145,14 +145,16 @@ def bench_stream(reps):
         log("bench: stream plain %s" % plain)
         untuned = _stream_line(s.run(h, form=kma_amd.FORM_BATCHED, nwg=1024, per_thread=1, warmup=1, reps=reps))
         log("bench: stream persistent, default heap %s" % untuned)
-    limit = int(plain["nulls"] * 1.05) + 1
+    # the persistent form may not buy speed with NULLs: no more than the
+    # per-launch form returned
+    limit = plain["nulls"]
     tune = s.autotune(STREAM_HEAP, 14, null_limit=limit, reps=2)
-    log("bench: autotuner picked sb_log2 %d table %d nwg %d per_thread %d (%.3e ops/s, nulls %d) of %d trials"
-        % (tune["config"]["sb_log2"], tune["table"], tune["nwg"], tune["per_thread"], tune["ops_per_s"],
+    log("bench: autotuner picked sb_log2 %d table %d agg %#x nwg %d per_thread %d (%.3e ops/s, nulls %d) of %d trials"
+        % (tune["config"]["sb_log2"], tune["table"], tune["agg"], tune["nwg"], tune["per_thread"], tune["ops_per_s"],
            tune["nulls"], len(tune["trials"])))
     with kma_amd.Heap.from_config(tune["cfg"]) as h:
         batched = _stream_line(s.run(h, form=kma_amd.FORM_BATCHED, nwg=tune["nwg"],
-                                     per_thread=tune["per_thread"], warmup=1, reps=reps))
+                                     per_thread=tune["per_thread"], warmup=1, reps=reps, variant=tune["agg"]))
     log("bench: stream persistent, tuned %s" % batched)
     return {
         "workload": "4,096 batches, Zipf(1.2) over 1..65,536 requests, 40% frees of earlier batches' "
@@ -168,8 +170,8 @@ def bench_stream(reps):
         "null_limit": limit,
         "tuned": {"sb_log2": tune["config"]["sb_log2"], "classes": tune["config"]["classes"],
                   "sblocks": tune["config"]["sblocks"], "npages": tune["config"]["npages"],
-                  "nwg": tune["nwg"], "per_thread": tune["per_thread"]},
-        "trials": [{k: t[k] for k in ("sb_log2", "table", "nwg", "per_thread", "ops_per_s", "nulls", "violations")}
+                  "agg": tune["agg"], "nwg": tune["nwg"], "per_thread": tune["per_thread"]},
+        "trials": [{k: t[k] for k in ("sb_log2", "table", "agg", "nwg", "per_thread", "ops_per_s", "nulls", "violations")}
                    for t in tune["trials"]],
     }
 

@@ -253,6 +253,9 @@ __device__ __forceinline__ char *sbdata(const kma_dheap *H, uint64_t sb) { retur
 __device__ __forceinline__ uint64_t *stats(const kma_dheap *H) { return (uint64_t *)(base(H) + H->off_stats); }
 __device__ __forceinline__ uint64_t *pmask(const kma_dheap *H, uint32_t x) { return (uint64_t *)(base(H) + H->off_stats + 256u + 128u * x); }
 __device__ __forceinline__ uint64_t *creq(const kma_dheap *H, uint32_t x) { return pmask(H, x) + 1; }
+/* wall-clock tick until which XCD x's pops skip the search beyond its own
+ * spares and shards (set by a search that found nothing) */
+__device__ __forceinline__ uint64_t *exhausted(const kma_dheap *H, uint32_t x) { return pmask(H, x) + 2; }
 __device__ __forceinline__ uint64_t psum_words(const kma_dheap *H) { return (((H->npages + 63) >> 6) + 63) >> 6; }
 __device__ __forceinline__ uint64_t *psum(const kma_dheap *H, uint32_t l) { return (uint64_t *)(base(H) + H->off_psum) + l * psum_words(H); }
 __device__ __forceinline__ uint64_t psum2_words(const kma_dheap *H) { return (psum_words(H) + 63) >> 6; }
@@ -329,11 +332,16 @@ __device__ __forceinline__ uint64_t shard_bit(const kma_dheap *H, uint32_t shard
 	return (uint64_t)1 << (shard % shards_per_xcd(H));
 }
 
-/* The shard went (or was found) empty: clear its hint unless a push got in. */
+/* The shard went (or was found) empty: clear its hint unless a push got in.
+ * A hint already clear is left alone (read first: under exhaustion many
+ * poppers find the same shard empty, and an RMW each would queue them all
+ * on the XCD's mask word). */
 __device__ inline void shard_emptied(const kma_dheap *H, uint32_t shard)
 {
 	uint64_t *mp = pmask(H, shard / shards_per_xcd(H));
 	uint64_t bit = shard_bit(H, shard);
+	if (!(ld(mp) & bit))
+		return;
 	f_and(mp, ~bit);
 	order_prior();
 	if ((uint32_t)ld(pool(H, shard)) != 0)
@@ -563,6 +571,15 @@ __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint3
  * caller fails at once. */
 #define KMA_CREQ_POLLS 1u
 #define KMA_STEAL_CTX 8u
+/* After a search that found no superblock anywhere, XCD x's pops give up
+ * after the caller's spares, x's own shards and a few other contexts'
+ * spares on x, for this many wall-clock ticks (10 ns each): under
+ * exhaustion every malloc would otherwise repeat the whole search (every
+ * XCD's masks and clean shards, a migration request and a poll), and those
+ * searches, all at once, queue on the same few words. */
+#ifndef KMA_EXHAUSTED_TICKS
+#define KMA_EXHAUSTED_TICKS 5000u
+#endif
 __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t salt, uint32_t *out)
 {
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
@@ -571,6 +588,7 @@ __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, ui
 	uint32_t sh = ctx + salt;
 	uint64_t mx = ld(pmask(H, x));
 	uint64_t rq = ld(creq(H, x));
+	uint64_t ex = ld(exhausted(H, x));
 	uint32_t occ = H->spare_slots ? spare_occupied(H, ctx) : 0u;
 	serve_creq(H, x, ctx, rq);
 	if (occ && spare_take(H, ctx, occ, salt, out))
@@ -592,35 +610,47 @@ __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, ui
 			if (o[i] && spare_take(H, x * per + (ctx - x * per + 1u + (c0 + i) % (per - 1u)) % per, o[i], salt, out))
 				return true;
 	}
+	if (ex && (uint64_t)wall_clock64() < ex)
+		return false;
+	/* m[j]: the mask of XCD (x + j) mod 8 (constant indices keep the
+	 * array in registers) */
 	uint64_t m[KMA_NXCD];
 	for (uint32_t poll = 0;; poll++) {
 #pragma unroll
-		for (uint32_t y = 0; y < KMA_NXCD; y++)
-			m[y] = ld(pmask(H, y));
+		for (uint32_t j = 0; j < KMA_NXCD; j++)
+			m[j] = ld(pmask(H, (x + j) & (KMA_NXCD - 1u)));
 		if (poll > 0) {
 			serve_creq(H, x, ctx, ld(creq(H, x)));
-			if (pool_pop_bits(H, x, m[x] & clean, D + sh % (S - D), out))
+			if (pool_pop_bits(H, x, m[0] & clean, D + sh % (S - D), out))
 				return true;
-			if (pool_pop_bits(H, x, m[x] & dirty, sh % D, out))
-				return true;
-		}
-		for (uint32_t j = 1; j < KMA_NXCD; j++) {
-			uint32_t y = (x + j) & (KMA_NXCD - 1u);
-			if (pool_pop_bits(H, y, m[y] & clean, D + sh % (S - D), out))
+			if (pool_pop_bits(H, x, m[0] & dirty, sh % D, out))
 				return true;
 		}
+#pragma unroll
+		for (uint32_t j = 1; j < KMA_NXCD; j++)
+			if (pool_pop_bits(H, (x + j) & (KMA_NXCD - 1u), m[j] & clean, D + sh % (S - D), out))
+				return true;
 		if (poll == KMA_CREQ_POLLS)
-			return false;
+			break;
 		if (poll == 0) {
+			/* ask the XCDs holding dirty superblocks for a migration; a
+			 * request already pending (read first, so that failing
+			 * callers do not all RMW the same words) means they have not
+			 * answered yet, and this caller fails at once */
 			uint32_t asked = 0;
-			for (uint32_t y = 0; y < KMA_NXCD; y++)
-				if (y != x && (m[y] & dirty) && !xchg(creq(H, y), (uint64_t)1))
+#pragma unroll
+			for (uint32_t j = 1; j < KMA_NXCD; j++) {
+				uint64_t *rw = creq(H, (x + j) & (KMA_NXCD - 1u));
+				if ((m[j] & dirty) && ld(rw) == 0 && !xchg(rw, (uint64_t)1))
 					asked++;
+			}
 			if (asked == 0)
-				return false;
+				break;
 		}
 		__builtin_amdgcn_s_sleep(32);
 	}
+	st(exhausted(H, x), (uint64_t)(wall_clock64() + KMA_EXHAUSTED_TICKS));
+	return false;
 }
 
 /* Called by whoever produced state `s` == {count 0, not owned, class set},
@@ -891,6 +921,10 @@ __device__ __forceinline__ void psum_mark_full(const kma_dheap *H, uint32_t l, u
 {
 	uint64_t *sp = psum(H, l) + (wi >> 6);
 	uint64_t bit = 1ull << (wi & 63);
+	/* already set: its setter re-checks the word (read first: many callers
+	 * find the same full word, and an RMW each would queue them) */
+	if (ld(sp) & bit)
+		return;
 	uint64_t old = f_or(sp, bit);
 	if (old & bit)
 		return;
@@ -907,9 +941,16 @@ __device__ __forceinline__ void psum_mark_full(const kma_dheap *H, uint32_t l, u
 __device__ __forceinline__ void psum_gained(const kma_dheap *H, uint64_t wi, uint64_t before, uint64_t after)
 {
 	uint64_t valid = page_valid(H, wi);
+	const uint64_t bit = (uint64_t)1 << (wi & 63);
 	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++)
-		if (!page_windows(before, l, valid) && page_windows(after, l, valid))
-			psum2_gained(H, l, wi >> 6, f_and(psum(H, l) + (wi >> 6), ~((uint64_t)1 << (wi & 63))));
+		if (!page_windows(before, l, valid) && page_windows(after, l, valid)) {
+			/* a bit read clear needs no clear: a setter whose set lands
+			 * after this read re-reads the word after this caller's
+			 * bitmap update and clears it itself */
+			uint64_t *sp = psum(H, l) + (wi >> 6);
+			if (ld(sp) & bit)
+				psum2_gained(H, l, wi >> 6, f_and(sp, ~bit));
+		}
 }
 
 /* Set the hint of every level that lost its last window: the protocol of
@@ -922,6 +963,8 @@ __device__ __forceinline__ void psum_lost(const kma_dheap *H, uint64_t wi, uint6
 	for (uint32_t l = 0; l < KMA_PAGE_LEVELS; l++) {
 		if (!page_windows(before, l, valid) || page_windows(after, l, valid))
 			continue;
+		if (ld(psum(H, l) + (wi >> 6)) & bit)
+			continue; /* already set: its setter re-checks the word */
 		uint64_t old = f_or(psum(H, l) + (wi >> 6), bit);
 		if (!(old & bit)) {
 			set |= 1u << l;
@@ -942,6 +985,15 @@ __device__ __forceinline__ void psum_lost(const kma_dheap *H, uint64_t wi, uint6
 	}
 }
 
+/* A request examines at most this many candidate bitmap words (words the
+ * summaries say may hold a window) before it gives up.  On a nearly full
+ * page region the few windows left draw every caller at once; beyond the
+ * first few candidates a caller mostly finds words that others just
+ * filled, and returns NULL after a bounded number of round trips instead of
+ * walking every summary word. */
+#ifndef KMA_PAGE_TRIES
+#define KMA_PAGE_TRIES 2u
+#endif
 __device__ __forceinline__ void *page_alloc(const kma_dheap *H, uint64_t size, uint32_t seed)
 {
 	uint64_t psz = 1ull << H->page_log2;
@@ -958,6 +1010,7 @@ __device__ __forceinline__ void *page_alloc(const kma_dheap *H, uint64_t size, u
 	uint32_t h = seed * 0x9E3779B1u;
 	const uint64_t si0 = (uint64_t)(h >> 8) % ns;
 	const uint32_t rot = h & 63u, rot2 = (uint32_t)(si0 & 63u);
+	uint32_t tries = 0;
 	waiter wt;
 	for (uint64_t jp = 0; jp < ns2; jp++) {
 		uint64_t t = (si0 >> 6) + jp;
@@ -985,6 +1038,8 @@ __device__ __forceinline__ void *page_alloc(const kma_dheap *H, uint64_t size, u
 					cand &= ~(1ull << b);
 					continue;
 				}
+				if (++tries > KMA_PAGE_TRIES)
+					return nullptr;
 				uint64_t valid = page_valid(H, wi);
 				uint64_t w = ld(pb + wi);
 				for (;;) {
@@ -1205,21 +1260,24 @@ __device__ inline uint64_t kma_pooled_superblocks(const kma_dheap *H)
  * in each bitmap word of a class's reserved slots sets that word's bits with
  * one RMW for all of them; a lane whose slot was already taken scans for
  * another free bit.  Lanes left without a slot (a busy head, a superblock
- * that filled up) go round again. */
+ * that filled up) go round again.  Lanes with no small request (size 0, or a
+ * page run served on their own) stay with the wavefront until it leaves the
+ * loop together. */
 __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 {
 	using namespace kma_nat;
-	if (size == 0)
-		return nullptr;
+	const bool small = size != 0 && size <= H->max_small;
+	char *result = nullptr;
 	if (size > H->max_small) {
 		uint32_t seed = (blockIdx.x * blockDim.x + threadIdx.x) ^ (uint32_t)clock64();
-		return page_alloc(H, size, seed);
+		result = (char *)page_alloc(H, size, seed);
 	}
-	const uint32_t c = (uint32_t)class_of(H, (uint32_t)size);
+	if (!__ballot(small))
+		return result;
+	const uint32_t c = small ? (uint32_t)class_of(H, (uint32_t)size) : 0u;
 	const uint32_t x = xcc();
 	const uint32_t ctx = ctx_of(H, x);
-	char *result = nullptr;
-	bool pending = true;
+	bool pending = small;
 	waiter wt;
 	for (;;) {
 		uint64_t left = __ballot(pending);
@@ -1272,25 +1330,27 @@ __device__ __forceinline__ void *kma_malloc(kma_dheap *H, size_t size)
 }
 
 /* Free a block obtained from kma_malloc on the same heap; NULL is a no-op.
- * A second free of the same block is detected and counted, not applied. */
+ * A second free of the same block is detected and counted, not applied.
+ * Lanes with no superblock block (NULL, a page run) stay with the
+ * wavefront until it leaves together. */
 __device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 {
 	using namespace kma_nat;
-	if (!ptr)
-		return;
 	char *p = (char *)ptr;
 	char *d0 = base(H) + H->off_data;
 	uint64_t off = (uint64_t)(p - d0);
-	if (p < d0 || off >= (H->nsb << H->sb_log2)) {
+	const bool small = p && p >= d0 && off < (H->nsb << H->sb_log2);
+	if (p && !small) {
 		if (H->npages && p >= base(H) + H->off_pages)
 			page_free(H, p);
 		else
 			bump_stat(H, KMA_STAT_BAD_FREE);
-		return;
 	}
+	if (!__ballot(small))
+		return;
 	uint32_t x = xcc();
-	uint32_t sb = (uint32_t)(off >> H->sb_log2);
-	uint64_t s = ld(state(H, sb));
+	uint32_t sb = small ? (uint32_t)(off >> H->sb_log2) : 0u;
+	uint64_t s = small ? ld(state(H, sb)) : 0u;
 	uint32_t c1 = st_cls1(s);
 	bool live = c1 != 0;
 	/* the caller's stores to the block complete before its bit is
@@ -1304,7 +1364,7 @@ __device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 	} else {
 		order_prior();
 	}
-	if (!live)
+	if (small && !live)
 		bump_stat(H, KMA_STAT_BAD_FREE);
 	uint32_t idx = 0;
 	if (live) {

@@ -5,17 +5,27 @@
  * kma_wg_malloc / kma_wg_free are called by EVERY thread of a 1-D work-group
  * (uniform control flow, like the reference's WG-collective
  * clArrayList_grow_local, clArrayList.cl:111-176).  Each thread passes its own
- * request (size 0 / NULL: none) and gets its own result; the work-group
- * stages the batch in LDS so that the device-wide atomics are issued per
- * size class and per bitmap word instead of per request:
- *   malloc: LDS class histogram -> one reservation (atomicAdd on the
- *           superblock state word) per class; a 16-lane group per class
- *           (all classes of the batch at once) reads the superblock bitmap
- *           coalesced (lane i <- word i), prefix-sums the free bits and claims
- *           them with one atomicOr per word.
- *   free:   an LDS hash keyed by (superblock, bitmap word) ORs the bits of all
- *           requests that share a word; one atomicAnd per word and one
- *           atomicSub on the state word per entry.
+ * request (size 0 / NULL: none) and gets its own result.  The aggregation
+ * width AGG (a template argument) says over how many requests the
+ * device-wide atomics are combined:
+ *   KMA_AGG_WAVE (default): each wavefront serves its own lanes with the
+ *           wavefront algorithm of kma_malloc / kma_free (lanes grouped by
+ *           class with ballots, one reservation per class group, one bitmap
+ *           RMW per word of consecutive reserved slots); no LDS, no barrier,
+ *           so the wavefronts of a work-group never wait for each other;
+ *   G = 8, 16 or 32: the work-group stages the whole batch in LDS:
+ *     malloc: LDS class histogram -> one reservation (atomicAdd on the
+ *             superblock state word) per class; a G-lane group per class
+ *             (all classes of the batch at once) reads the superblock bitmap
+ *             coalesced (lane i <- word i), prefix-sums the free bits and
+ *             claims them with one atomicOr per word;
+ *     free:   an LDS hash keyed by (superblock, bitmap word) ORs the bits of
+ *             all requests that share a word; one atomicAnd per word and one
+ *             atomicSub on the state word per entry.
+ *   Work-group staging combines more requests per atomic (a class with a
+ *   superblock's worth of requests takes whole superblocks) at the price of
+ *   barriers; which width wins depends on the batch, and the stream
+ *   autotuner picks it.
  * Requests larger than the largest class go to the page allocator per lane.
  */
 #ifndef KMA_WG_DEVICE_H
@@ -48,28 +58,29 @@ __device__ __forceinline__ uint32_t low_bits(uint32_t m, uint32_t k)
 	return r;
 }
 
-/* ---- 16-lane groups: one size class per group, four per wavefront ----
+/* ---- G-lane groups: one size class per group, 64/G per wavefront ----
  * Every lane of the wavefront executes these (the loops are
  * wavefront-uniform); a group with nothing to do passes dummies. */
-#define KMA_CLAIM_G 16u
+template <uint32_t G>
+__device__ __forceinline__ uint32_t grp_lane() { return lane() & (G - 1u); }
 
-__device__ __forceinline__ uint32_t grp_lane() { return lane() & (KMA_CLAIM_G - 1u); }
-
+template <uint32_t G>
 __device__ __forceinline__ uint32_t group_excl_scan(uint32_t v, uint32_t *total)
 {
-	uint32_t x = v, gl = grp_lane();
-	for (uint32_t o = 1; o < KMA_CLAIM_G; o <<= 1) {
-		uint32_t y = __shfl_up(x, o, (int)KMA_CLAIM_G);
+	uint32_t x = v, gl = grp_lane<G>();
+	for (uint32_t o = 1; o < G; o <<= 1) {
+		uint32_t y = __shfl_up(x, o, (int)G);
 		if (gl >= o)
 			x += y;
 	}
-	*total = __shfl(x, (int)KMA_CLAIM_G - 1, (int)KMA_CLAIM_G);
+	*total = __shfl(x, (int)G - 1, (int)G);
 	return x - v;
 }
 
+template <uint32_t G>
 __device__ __forceinline__ uint32_t group_bcast(uint32_t v)
 {
-	return __shfl(v, 0, (int)KMA_CLAIM_G);
+	return __shfl(v, 0, (int)G);
 }
 
 /* Each group with want set claims `got` free bits of its superblock sb
@@ -80,10 +91,11 @@ __device__ __forceinline__ uint32_t group_bcast(uint32_t v)
  * RMWs say which were).  Later passes read the words from the one holding
  * bit `base`, 16 words per pass, and take the lowest free bits; after a full
  * pass over the bitmap without progress the words are re-read with RMWs. */
+template <uint32_t G>
 __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t sb, uint32_t c, uint32_t base,
 				       uint32_t got, const uint16_t *order, uint64_t *res)
 {
-	const uint32_t gl = grp_lane();
+	const uint32_t gl = grp_lane<G>();
 	uint32_t cap = want ? H->class_cap[c] : 32u;
 	uint32_t W = (cap + 31u) >> 5;
 	uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
@@ -108,27 +120,27 @@ __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t s
 				freeb &= tail;
 		}
 		uint32_t tot;
-		uint32_t pre = group_excl_scan(__popc(freeb), &tot);
+		uint32_t pre = group_excl_scan<G>(__popc(freeb), &tot);
 		uint32_t take = need ? min((uint32_t)__popc(freeb), sat_sub(got - done, pre)) : 0u;
 		uint32_t mask = low_bits(freeb, take);
 		uint32_t won = 0;
 		if (mask)
 			won = mask & ~f_or(bm + wi, mask);
 		uint32_t wtot;
-		uint32_t wpre = group_excl_scan(__popc(won), &wtot);
+		uint32_t wpre = group_excl_scan<G>(__popc(won), &wtot);
 		uint32_t r = done + wpre;
 		while (won) {
 			uint32_t b = __ffs(won) - 1u;
 			won &= won - 1u;
 			res[order[r++]] = d0 + (uint64_t)(wi * 32u + b) * bsize;
 		}
-		const bool exp = group_bcast(need && wt.expired() ? 1u : 0u) != 0;
+		const bool exp = group_bcast<G>(need && wt.expired() ? 1u : 0u) != 0;
 		if (need) {
 			done += wtot;
 			if (opt)
 				opt = false; /* then scan from word w0 */
 			else
-				chunk += KMA_CLAIM_G;
+				chunk += G;
 			if (chunk >= W) {
 				/* a full pass: re-read with RMWs if it made no progress */
 				chunk = 0;
@@ -153,11 +165,12 @@ __device__ inline uint32_t group_claim(const kma_dheap *H, bool want, uint32_t s
  * slot that does not (a corrupt heap) is left out and its count given back,
  * and its rank gets NULL. */
 #define KMA_WHOLE_WORDS 8u
+template <uint32_t G>
 __device__ __forceinline__ uint32_t whole_superblocks(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c,
 					     uint32_t cap, bool want, const uint16_t *order, uint64_t *res)
 {
 	uint32_t sb = 0;
-	bool have = want && pool_pop(H, x, ctx, grp_lane(), &sb);
+	bool have = want && pool_pop(H, x, ctx, grp_lane<G>(), &sb);
 	uint32_t old[KMA_WHOLE_WORDS];
 	const uint32_t W = (cap + 31u) >> 5;
 	const uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
@@ -170,7 +183,7 @@ __device__ __forceinline__ uint32_t whole_superblocks(const kma_dheap *H, uint32
 				old[i] = f_or(bm + i, i == W - 1u ? tail : ~0u);
 	}
 	uint32_t tot;
-	const uint32_t idx = group_excl_scan(have ? 1u : 0u, &tot);
+	const uint32_t idx = group_excl_scan<G>(have ? 1u : 0u, &tot);
 	if (have) {
 		const uint64_t d0 = H->off_data + ((uint64_t)sb << H->sb_log2);
 		const uint32_t bsize = H->class_size[c];
@@ -201,13 +214,23 @@ __device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
 
 } /* namespace kma_nat */
 
+/* Aggregation widths (see the top of this file). */
+#define KMA_AGG_WAVE 0u
+
 /* Work-group collective malloc.  All threads of the 1-D work-group call it;
- * size 0 means "no request" for that thread.  per_lane_claim selects one
- * bitmap atomic per block instead of one per bitmap word. */
-template <int BLOCK>
+ * size 0 means "no request" for that thread.  With work-group aggregation,
+ * per_lane_claim selects one bitmap atomic per block instead of one per
+ * bitmap word. */
+template <int BLOCK, uint32_t AGG = KMA_AGG_WAVE>
 __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg_lds<BLOCK> &L, uint32_t per_lane_claim = 0)
 {
 	using namespace kma_nat;
+	static_assert(AGG == KMA_AGG_WAVE || AGG == 8u || AGG == 16u || AGG == 32u, "aggregation width");
+	if constexpr (AGG == KMA_AGG_WAVE) {
+		(void)L;
+		(void)per_lane_claim;
+		return kma_malloc(H, size);
+	}
 	const uint32_t tid = threadIdx.x, nth = blockDim.x, nw = (nth + 63u) >> 6;
 	const uint32_t x = xcc(), ctx = wg_ctx_of(H, x);
 	const uint32_t ncls = H->nclasses;
@@ -236,7 +259,8 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 	/* one 16-lane group per class, all classes at once: the group's first
 	 * lane reserves a segment, the group claims it; repeat until the class is
 	 * served or the heap cannot serve it (loops are wavefront-uniform) */
-	const uint32_t G = KMA_CLAIM_G, ngroups = nw * (64u / G), g = tid / G, gl = tid % G;
+	constexpr uint32_t G = AGG ? AGG : 16u;
+	const uint32_t ngroups = nw * (64u / G), g = tid / G, gl = tid % G;
 	const uint32_t iters = (ncls + ngroups - 1u) / ngroups;
 	for (uint32_t it = 0; it < iters; it++) {
 		const uint32_t k = g + it * ngroups;
@@ -249,7 +273,7 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 		const uint32_t cap = n ? H->class_cap[k] : 0u;
 		const uint32_t nwhole = (n && cap > 1u && cap <= 32u * KMA_WHOLE_WORDS) ? min(n / cap, G) : 0u;
 		if (__ballot(nwhole != 0u))
-			r0 = whole_superblocks(H, x, ctx, k, cap, gl < nwhole, ord, L.res);
+			r0 = whole_superblocks<G>(H, x, ctx, k, cap, gl < nwhole, ord, L.res);
 		bool pend = r0 < n;
 		waiter wt;
 		while (__ballot(pend)) {
@@ -262,8 +286,8 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 			resv r = {0, 0, 0, 0};
 			if (pend && gl == 0)
 				r = reserve(H, x, ctx, k, n - r0);
-			const uint32_t ok = group_bcast(r.ok), sb = group_bcast(r.sb);
-			const uint32_t rb = group_bcast(r.base), got = group_bcast(r.got);
+			const uint32_t ok = group_bcast<G>(r.ok), sb = group_bcast<G>(r.sb);
+			const uint32_t rb = group_bcast<G>(r.base), got = group_bcast<G>(r.got);
 			const bool want = pend && ok;
 			if (pend && !ok)
 				pend = false;
@@ -278,10 +302,10 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 						mine += p ? 1u : 0u;
 					}
 				uint32_t tot;
-				group_excl_scan(mine, &tot);
+				group_excl_scan<G>(mine, &tot);
 				done = tot;
 			} else {
-				done = group_claim(H, want, sb, k, rb, got, ord + r0, L.res);
+				done = group_claim<G>(H, want, sb, k, rb, got, ord + r0, L.res);
 			}
 			if (want) {
 				r0 += done;
@@ -312,10 +336,15 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 
 /* Work-group collective free.  All threads call it; NULL means "no
  * request".  Double frees are detected and counted, not applied. */
-template <int BLOCK>
+template <int BLOCK, uint32_t AGG = KMA_AGG_WAVE>
 __device__ __forceinline__ void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<BLOCK> &L)
 {
 	using namespace kma_nat;
+	if constexpr (AGG == KMA_AGG_WAVE) {
+		(void)L;
+		kma_free(H, ptr);
+		return;
+	}
 	const uint32_t tid = threadIdx.x, nth = blockDim.x;
 	const uint32_t x = xcc(), ctx = wg_ctx_of(H, x);
 	const uint32_t HN = 2u * BLOCK;

@@ -49,6 +49,16 @@ typedef struct kma_workload {
 #define KMA_VAR_LANE_MALLOC 0x1u   /* BATCHED: per-lane kma_malloc instead of kma_wg_malloc */
 #define KMA_VAR_LANE_FREE   0x2u   /* BATCHED: per-lane kma_free instead of kma_wg_free */
 #define KMA_VAR_LANE_CLAIM  0x4u   /* BATCHED: one bitmap atomic per block, not per word */
+/* BATCHED forms (workloads and streams): aggregation width of the
+ * collective calls (kma_wg_device.h).  WAVE: each wavefront serves its
+ * lanes (no LDS, no barrier); WG8/16/32: the work-group stages its
+ * requests in LDS and claims with 8-, 16- or 32-lane groups. */
+#define KMA_VAR_AGG_SHIFT   4u
+#define KMA_VAR_AGG_MASK    0x30u
+#define KMA_VAR_AGG_WAVE    0x00u
+#define KMA_VAR_AGG_WG8     0x10u
+#define KMA_VAR_AGG_WG16    0x20u
+#define KMA_VAR_AGG_WG32    0x30u
 #define KMA_VAR_LATENCY     0x100u /* record every malloc's latency (wall clock) */
 #define KMA_VAR_PHASES      0x200u /* time the malloc / hold / free phases */
 
@@ -202,8 +212,10 @@ typedef struct kma_stream_stats {
  * KMA_FORM_PLAIN:   one launch per batch, per-lane kma_malloc / kma_free.
  * KMA_FORM_BATCHED: ONE graph-captured persistent launch of nwg work-groups
  *   (256 threads) that claim chunks of 256 * per_thread consecutive requests
- *   and run each 256-request slice with kma_wg_malloc then kma_wg_free; a
- *   free waits for the result of its malloc (always an earlier request).
+ *   and run the chunk's 256-request slices through kma_wg_malloc, then
+ *   through kma_wg_free, at the aggregation width variant & KMA_VAR_AGG_MASK
+ *   selects; a free waits for the result of its malloc (always an earlier
+ *   request).
  * warmup untimed replays, then reps timed replays; the heap must be empty
  * before each replay (true after a draining replay). */
 int kma_stream_run(kma_heap *h, kma_stream *s, uint32_t form, uint32_t nwg, uint32_t per_thread,
@@ -220,9 +232,10 @@ int kma_stream_run(kma_heap *h, kma_stream *s, uint32_t form, uint32_t nwg, uint
 int kma_stream_heap_config(const kma_stream *s, uint64_t heap_bytes, uint32_t sb_log2, uint32_t table,
 			   uint32_t page_log2, kma_config *out);
 
-#define KMA_TUNE_MAX 64
+#define KMA_TUNE_MAX 128
 typedef struct kma_tune_trial {
 	uint32_t sb_log2, table, nwg, per_thread;
+	uint32_t agg;         /* aggregation width: a KMA_VAR_AGG_* value */
 	double ops_per_s;     /* requests per second */
 	double ms;            /* per replay */
 	uint64_t nulls;
@@ -234,16 +247,20 @@ typedef struct kma_tune_result {
 	uint32_t nwg, per_thread; /* chosen persistent launch shape */
 	uint32_t table;
 	uint32_t ntrials;
+	uint32_t agg;         /* chosen aggregation width (KMA_VAR_AGG_*): pass it
+				 in kma_stream_run's variant */
 	double ops_per_s;
 	uint64_t nulls;
 	kma_tune_trial trials[KMA_TUNE_MAX];
 } kma_tune_result;
 
 /* Autotuner for the persistent form: sweeps superblock size (16, 32,
- * 64 KiB), class table (KMA_TABLE_*), work-groups (512, 1024) and requests
- * per thread per chunk (1, 2, 4), `reps` timed replays each, and picks the
- * highest throughput among trials with no violation and at most `null_limit`
- * NULL returns (no limit when 0).  Every trial heap is `heap_bytes`. */
+ * 64 KiB), class table (KMA_TABLE_*), aggregation width (wavefront, or
+ * work-group staging with 8-, 16- or 32-lane claim groups), work-groups
+ * (512, 1024) and requests per thread per chunk (1, 2): 96 trials of `reps`
+ * timed replays each, and picks the highest throughput among trials with no
+ * violation and at most `null_limit` NULL returns (no limit when 0).  Every
+ * trial heap is `heap_bytes`. */
 int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, uint64_t null_limit,
 			uint32_t reps, kma_tune_result *out);
 

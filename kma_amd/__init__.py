@@ -42,7 +42,8 @@ EXPORTS = [
 DET_NOALLOC_MAX = 524288
 GRAPH_PLAIN, GRAPH_CACHE, GRAPH_COLLECTIVE = 0, 1, 2
 TABLE_POW2, TABLE_HALF = 0, 1
-TUNE_MAX = 64
+TUNE_MAX = 128
+VAR_AGG_WAVE, VAR_AGG_WG8, VAR_AGG_WG16, VAR_AGG_WG32 = 0x00, 0x10, 0x20, 0x30
 
 
 class Config(ctypes.Structure):
@@ -150,7 +151,8 @@ class StreamStats(ctypes.Structure):
 class TuneTrial(ctypes.Structure):
     _fields_ = [
         ("sb_log2", ctypes.c_uint32), ("table", ctypes.c_uint32), ("nwg", ctypes.c_uint32),
-        ("per_thread", ctypes.c_uint32), ("ops_per_s", ctypes.c_double), ("ms", ctypes.c_double),
+        ("per_thread", ctypes.c_uint32), ("agg", ctypes.c_uint32), ("ops_per_s", ctypes.c_double),
+        ("ms", ctypes.c_double),
         ("nulls", ctypes.c_uint64), ("violations", ctypes.c_uint64),
     ]
 
@@ -158,7 +160,8 @@ class TuneTrial(ctypes.Structure):
 class TuneResult(ctypes.Structure):
     _fields_ = [
         ("cfg", Config), ("nwg", ctypes.c_uint32), ("per_thread", ctypes.c_uint32),
-        ("table", ctypes.c_uint32), ("ntrials", ctypes.c_uint32), ("ops_per_s", ctypes.c_double),
+        ("table", ctypes.c_uint32), ("ntrials", ctypes.c_uint32), ("agg", ctypes.c_uint32),
+        ("ops_per_s", ctypes.c_double),
         ("nulls", ctypes.c_uint64), ("trials", TuneTrial * TUNE_MAX),
     ]
 
@@ -564,7 +567,7 @@ class Stream:
                                          ctypes.byref(r)), "kma_stream_autotune")
         trials = [{k: getattr(t, k) for k, _ in TuneTrial._fields_} for t in r.trials[:r.ntrials]]
         return {"cfg": r.cfg, "config": config_dict(r.cfg), "nwg": r.nwg, "per_thread": r.per_thread,
-                "table": r.table, "ops_per_s": r.ops_per_s, "nulls": r.nulls, "trials": trials}
+                "table": r.table, "agg": r.agg, "ops_per_s": r.ops_per_s, "nulls": r.nulls, "trials": trials}
 
 
 def splitmix(seed, gid, k):

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_wg(kma_dheap *H, kma_tree *t, 
 		const uint32_t ks = has ? edges[2 * i] : 0u, kk = has ? edges[2 * i + 1] : 0u;
 		gnode *src = has ? (gnode *)kma_tree_get(t, ks) : nullptr;
 		bool need = has && !src && !cache;
-		void *p = kma_wg_malloc<GBLOCK>(H, need ? sizeof(gnode) : 0u, L);
+		void *p = kma_wg_malloc<GBLOCK, 16u>(H, need ? sizeof(gnode) : 0u, L);
 		if (need) {
 			cache = (gnode *)p;
 			c.v[G_NODE_MALLOC]++;
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_wg(kma_dheap *H, kma_tree *t, 
 			src = ensure_from(t, ks, cache, c);
 		gnode *snk = has && src ? (gnode *)kma_tree_get(t, kk) : nullptr;
 		need = has && src && !snk && !cache;
-		p = kma_wg_malloc<GBLOCK>(H, need ? sizeof(gnode) : 0u, L);
+		p = kma_wg_malloc<GBLOCK, 16u>(H, need ? sizeof(gnode) : 0u, L);
 		if (need) {
 			cache = (gnode *)p;
 			c.v[G_NODE_MALLOC]++;
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_wg(kma_dheap *H, kma_tree *t, 
 		if (has && src && !snk)
 			snk = ensure_from(t, kk, cache, c);
 		const bool linkit = has && src && snk;
-		glink *l = (glink *)kma_wg_malloc<GBLOCK>(H, linkit ? sizeof(glink) : 0u, L);
+		glink *l = (glink *)kma_wg_malloc<GBLOCK, 16u>(H, linkit ? sizeof(glink) : 0u, L);
 		if (linkit) {
 			c.v[G_LINK_MALLOC]++;
 			if (l) {
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_wg(kma_dheap *H, kma_tree *t, 
 		dead = dead || (i < n && !(linkit && l));
 	}
 	c.v[G_NODE_FREE] += cache ? 1u : 0u;
-	kma_wg_free<GBLOCK>(H, cache, L);
+	kma_wg_free<GBLOCK, 16u>(H, cache, L);
 	flush(stats, c);
 }
 

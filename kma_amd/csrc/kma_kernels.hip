@@ -304,28 +304,47 @@ extern "C" int kmad_atomic_peak(void *buf, uint64_t nwords, uint64_t nthreads, u
 /* batched persistent form                                              */
 /* ------------------------------------------------------------------ */
 /*
- * One launch runs every round; each work-group stages its requests in LDS
- * and allocates / frees them collectively (kma_wg_device.h).  Work-items
- * past nwi take part with empty requests so control flow stays uniform.
+ * One launch runs every round; the work-groups allocate and free their
+ * requests collectively (kma_wg_device.h) with the aggregation width AGG.
+ * Work-items past nwi take part with empty requests so control flow stays
+ * uniform.
  */
-template <bool GENERIC>
+template <bool GENERIC, uint32_t AGG>
 __global__ void __launch_bounds__(KMA_BATCH_BLOCK) k_batched(kma_dheap *H, kma_wl_args a)
 {
 	__shared__ kma_wg_lds<KMA_BATCH_BLOCK> L;
-	wl_batched<GENERIC>(H, L, a);
+	wl_batched<GENERIC, AGG>(H, L, a);
+}
+
+template <bool GENERIC>
+static void launch_batched(uint32_t agg, uint64_t blocks, hipStream_t s, kma_dheap *H, const kma_wl_args &a)
+{
+	switch (agg) {
+	case 1:
+		hipLaunchKernelGGL((k_batched<GENERIC, 8u>), dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, s, H, a);
+		break;
+	case 2:
+		hipLaunchKernelGGL((k_batched<GENERIC, 16u>), dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, s, H, a);
+		break;
+	case 3:
+		hipLaunchKernelGGL((k_batched<GENERIC, 32u>), dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, s, H, a);
+		break;
+	default:
+		hipLaunchKernelGGL((k_batched<GENERIC, KMA_AGG_WAVE>), dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0, s,
+				   H, a);
+	}
 }
 
 extern "C" int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode, uint32_t fixed, uint64_t seed,
 			    uint64_t nwi, uint32_t variant, unsigned long long *d_stats, uint32_t *d_lat, void *stream)
 {
 	uint64_t blocks = (nwi + KMA_BATCH_BLOCK - 1) / KMA_BATCH_BLOCK;
-	kma_wl_args a = {round0, nrounds, mode, fixed, seed, nwi, variant, d_stats, d_lat};
-	if (variant == 0 && !d_lat)
-		hipLaunchKernelGGL(k_batched<false>, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0,
-				   (hipStream_t)stream, (kma_dheap *)heap, a);
+	const uint32_t agg = (variant & KMA_VAR_AGG_MASK) >> KMA_VAR_AGG_SHIFT;
+	kma_wl_args a = {round0, nrounds, mode, fixed, seed, nwi, variant & ~KMA_VAR_AGG_MASK, d_stats, d_lat};
+	if (a.variant == 0 && !d_lat)
+		launch_batched<false>(agg, blocks, (hipStream_t)stream, (kma_dheap *)heap, a);
 	else
-		hipLaunchKernelGGL(k_batched<true>, dim3((uint32_t)blocks), dim3(KMA_BATCH_BLOCK), 0,
-				   (hipStream_t)stream, (kma_dheap *)heap, a);
+		launch_batched<true>(agg, blocks, (hipStream_t)stream, (kma_dheap *)heap, a);
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }
@@ -342,7 +361,7 @@ __global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, uint32_t 
 {
 	__shared__ uint16_t order[4][64];
 	__shared__ uint64_t res[4][64];
-	const uint32_t l = threadIdx.x, g = l / KMA_CLAIM_G;
+	const uint32_t l = threadIdx.x, g = l / 16u;
 	for (uint32_t q = 0; q < 4; q++) {
 		uint32_t cs = blockIdx.x * 4u + q;
 		if (cs < ncases)
@@ -354,7 +373,7 @@ __global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, uint32_t 
 	__syncthreads();
 	const uint32_t cs = blockIdx.x * 4u + g;
 	const bool want = cs < ncases;
-	uint32_t done = kma_nat::group_claim(H, want, cs, want ? cls[cs] : 0u, want ? base[cs] : 0u,
+	uint32_t done = kma_nat::group_claim<16u>(H, want, cs, want ? cls[cs] : 0u, want ? base[cs] : 0u,
 					     want ? got[cs] : 0u, order[g], res[g]);
 	__syncthreads();
 	for (uint32_t q = 0; q < 4; q++) {
@@ -365,7 +384,7 @@ __global__ void __launch_bounds__(64) k_check_wave_claim(kma_dheap *H, uint32_t 
 		for (uint32_t w = l; w < H->bm_words; w += 64)
 			bm_out[c2 * H->bm_words + w] = kma_nat::ld(kma_nat::bitmap(H, c2) + w);
 	}
-	if (want && kma_nat::grp_lane() == 0)
+	if (want && kma_nat::grp_lane<16u>() == 0)
 		done_out[cs] = done;
 }
 

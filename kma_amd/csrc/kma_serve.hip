@@ -48,6 +48,10 @@
 	} while (0)
 
 #define SV_BLOCK 256
+/* aggregation width of the loop's collective calls: work-group staging
+ * with 16-lane claim groups (every work-group asks for 256 blocks of one
+ * class per step, which whole-superblock grabs serve) */
+#define SV_AGG 16u
 
 namespace {
 
@@ -302,7 +306,7 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 			const int64_t job = over ? -1 : job0;
 			const bool live = job >= 0;
 			const uint64_t t = wall_clock64();
-			char *p = (char *)kma_wg_malloc<SV_BLOCK>(H, live ? a.block_bytes : 0u, L);
+			char *p = (char *)kma_wg_malloc<SV_BLOCK, SV_AGG>(H, live ? a.block_bytes : 0u, L);
 			const uint64_t dt = wall_clock64() - t;
 			const int nl = __syncthreads_count(live);
 			const int ns = __syncthreads_count(live && !p);
@@ -358,7 +362,7 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 					atomicAdd((unsigned long long *)&a.result[job], (unsigned long long)v);
 				c_frees++;
 			}
-			kma_wg_free<SV_BLOCK>(H, q, L);
+			kma_wg_free<SV_BLOCK, SV_AGG>(H, q, L);
 		}
 		if (tid < 64) {
 			uint64_t part = kma_free_superblocks_part(H, blockIdx.x, nwg);
@@ -387,7 +391,7 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 		for (uint32_t k = 0; k < m; k++) {
 			void *q = k < steps ? (void *)(uintptr_t)a.table[(uint64_t)s * a.max_len + k] : nullptr;
 			c_dfrees += q != nullptr;
-			kma_wg_free<SV_BLOCK>(H, q, L);
+			kma_wg_free<SV_BLOCK, SV_AGG>(H, q, L);
 		}
 		if (job >= 0)
 			a.slot_job[s] = -1;

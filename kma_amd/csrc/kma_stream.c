@@ -484,8 +484,9 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 			kma_tune_result *out)
 {
 	static const uint32_t sbs[] = {14, 15, 16};
+	static const uint32_t aggs[] = {KMA_VAR_AGG_WAVE, KMA_VAR_AGG_WG8, KMA_VAR_AGG_WG16, KMA_VAR_AGG_WG32};
 	static const uint32_t nwgs[] = {512, 1024};
-	static const uint32_t pts[] = {1, 2, 4};
+	static const uint32_t pts[] = {1, 2};
 	if (!s || !out || reps == 0) {
 		set_err("bad arguments");
 		return -1;
@@ -500,11 +501,12 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 			kma_heap *h = kma_create_ex(&cfg);
 			if (!h)
 				return -1;
+			for (uint32_t g = 0; g < sizeof(aggs) / sizeof(aggs[0]); g++) {
 			for (uint32_t b = 0; b < sizeof(nwgs) / sizeof(nwgs[0]); b++) {
 				for (uint32_t c = 0; c < sizeof(pts) / sizeof(pts[0]); c++) {
 					kma_stream_stats st;
 					if (out->ntrials >= KMA_TUNE_MAX ||
-					    kma_stream_run(h, s, KMA_FORM_BATCHED, nwgs[b], pts[c], 1, reps, 0, &st) != 0) {
+					    kma_stream_run(h, s, KMA_FORM_BATCHED, nwgs[b], pts[c], 1, reps, aggs[g], &st) != 0) {
 						kma_destroy(h);
 						return -1;
 					}
@@ -513,6 +515,7 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 					t->table = table;
 					t->nwg = nwgs[b];
 					t->per_thread = pts[c];
+					t->agg = aggs[g];
 					t->ms = st.gpu_ms;
 					t->ops_per_s = st.gpu_ms > 0 ? (double)st.ops / (st.gpu_ms * 1e-3) : 0;
 					t->nulls = st.nulls;
@@ -527,6 +530,7 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 					    (best < 0 || t->ops_per_s > out->trials[best].ops_per_s))
 						best = i;
 				}
+			}
 			}
 			kma_destroy(h);
 		}
@@ -543,6 +547,7 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 	out->nwg = t->nwg;
 	out->per_thread = t->per_thread;
 	out->table = t->table;
+	out->agg = t->agg;
 	out->ops_per_s = t->ops_per_s;
 	out->nulls = t->nulls;
 	return 0;

@@ -5,7 +5,8 @@
  *   k_stream_persistent one launch for the whole stream: work-groups claim
  *                       chunks of consecutive requests, serve the chunk's
  *                       mallocs slice by slice (256 requests) with
- *                       kma_wg_malloc, then its frees with kma_wg_free.
+ *                       kma_wg_malloc, then its frees with kma_wg_free, at
+ *                       the aggregation width the variant selects.
  *
  * Every malloc writes a pattern derived from its request index into the
  * block's first word and then publishes (epoch << 40 | heap offset) in the
@@ -196,6 +197,7 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_plain(kma_dheap *H, const uin
 	flush(ctl, c, ph);
 }
 
+template <uint32_t AGG>
 __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, const uint32_t *req, uint64_t *res,
 							      uint64_t nreq, uint32_t per_thread, uint64_t *ctl,
 							      uint64_t *ts, uint32_t variant)
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, cons
 			const bool in = j < nreq;
 			const uint32_t r = in ? req[j] : 0u;
 			const uint32_t size = (in && !(r >> 31)) ? r : 0u;
-			void *p = kma_wg_malloc<SBLOCK>(H, size, L);
+			void *p = kma_wg_malloc<SBLOCK, AGG>(H, size, L);
 			if (size) {
 				count_malloc(H, size, p, c);
 				if (p)
@@ -253,7 +255,7 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, cons
 					ts[j] = wall_clock64();
 			}
 			ph.mark(1);
-			kma_wg_free<SBLOCK>(H, q, L);
+			kma_wg_free<SBLOCK, AGG>(H, q, L);
 			ph.mark(2);
 		}
 		ph.n += nsl;
@@ -286,8 +288,25 @@ extern "C" int kmad_stream_persistent(void *heap, const uint32_t *req, uint64_t 
 {
 	if (nwg == 0 || per_thread == 0)
 		return -1;
-	hipLaunchKernelGGL(k_stream_persistent, dim3(nwg), dim3(SBLOCK), 0, (hipStream_t)stream, (kma_dheap *)heap,
-			   req, res, nreq, per_thread, ctl, ts, variant);
+	hipStream_t s = (hipStream_t)stream;
+	kma_dheap *H = (kma_dheap *)heap;
+	switch ((variant & KMA_VAR_AGG_MASK) >> KMA_VAR_AGG_SHIFT) {
+	case 1:
+		hipLaunchKernelGGL(k_stream_persistent<8u>, dim3(nwg), dim3(SBLOCK), 0, s, H, req, res, nreq, per_thread,
+				   ctl, ts, variant);
+		break;
+	case 2:
+		hipLaunchKernelGGL(k_stream_persistent<16u>, dim3(nwg), dim3(SBLOCK), 0, s, H, req, res, nreq, per_thread,
+				   ctl, ts, variant);
+		break;
+	case 3:
+		hipLaunchKernelGGL(k_stream_persistent<32u>, dim3(nwg), dim3(SBLOCK), 0, s, H, req, res, nreq, per_thread,
+				   ctl, ts, variant);
+		break;
+	default:
+		hipLaunchKernelGGL(k_stream_persistent<KMA_AGG_WAVE>, dim3(nwg), dim3(SBLOCK), 0, s, H, req, res, nreq,
+				   per_thread, ctl, ts, variant);
+	}
 	KMA_CHECK(hipGetLastError());
 	return 0;
 }

@@ -224,7 +224,7 @@ __device__ inline void wl_rounds(HeapT *heap, const kma_wl_args &a)
 /* GENERIC = false compiles the plain collective path only (variant 0, no
  * latency probes): the per-lane malloc/free and the probes would otherwise
  * set the register budget, and with it the occupancy, of the whole kernel. */
-template <bool GENERIC>
+template <bool GENERIC, uint32_t AGG>
 __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, const kma_wl_args &a)
 {
 	const uint32_t round0 = a.round0, nrounds = a.nrounds, fixed = a.fixed;
@@ -253,7 +253,7 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 		if (variant & 1u)
 			p = (uint32_t *)kma_malloc(H, size);
 		else
-			p = (uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK>(H, size, L, (variant >> 2) & 1u);
+			p = (uint32_t *)kma_wg_malloc<KMA_BATCH_BLOCK, AGG>(H, size, L, (variant >> 2) & 1u);
 		if (lat && active)
 			lat[(uint64_t)(r - round0) * nwi + gid] = (uint32_t)(wall_clock64() - t0);
 		ph.mark(0);
@@ -284,7 +284,7 @@ __device__ inline void wl_batched(kma_dheap *H, kma_wg_lds<KMA_BATCH_BLOCK> &L, 
 		if (variant & 2u)
 			kma_free(H, (void *)p);
 		else
-			kma_wg_free<KMA_BATCH_BLOCK>(H, (void *)p, L);
+			kma_wg_free<KMA_BATCH_BLOCK, AGG>(H, (void *)p, L);
 		ph.mark(2);
 	}
 	wave_add_stats(stats, ops, nulls, bad, mallocs);

@@ -171,7 +171,19 @@ int main(int argc, char **argv)
 				g_sim_lane.wg = wgs[b].get();
 				g_sim_lane.wave = waves[b * (KMA_BATCH_BLOCK / 64) + t / 64].get();
 				kma_wl_args wa = {0, rounds, mode, 64, seed, nwi, variant, stats, nullptr};
-				wl_batched<true>(H, *lds[b], wa);
+				switch ((variant & KMA_VAR_AGG_MASK) >> KMA_VAR_AGG_SHIFT) {
+				case 1:
+					wl_batched<true, 8u>(H, *lds[b], wa);
+					break;
+				case 2:
+					wl_batched<true, 16u>(H, *lds[b], wa);
+					break;
+				case 3:
+					wl_batched<true, 32u>(H, *lds[b], wa);
+					break;
+				default:
+					wl_batched<true, KMA_AGG_WAVE>(H, *lds[b], wa);
+				}
 			});
 	for (auto &t : th)
 		t.join();

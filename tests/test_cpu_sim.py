@@ -34,14 +34,22 @@ def _run(sim, *args):
 
 @pytest.mark.parametrize("args", [
     # sb_log2 sblocks nwg nwi rounds mode seed nctx npool [variant]
-    (14, 1024, 4, 1024, 3, 1, 1, 64, 16),     # one context per work-group
-    (14, 65536, 16, 4096, 4, 1, 1, 8, 16),    # two work-groups per context
-    (12, 4096, 4, 1000, 20, 0, 1, 64, 64),    # fixed 64 B, partial last work-group
-    (12, 4096, 4, 777, 10, 1, 5, 64, 64),     # 4 KiB superblocks, mixed classes
-    (14, 64, 8, 2048, 3, 1, 9, 8, 16),        # tiny heap: pool exhaustion, NULLs allowed
-    (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 2), # collective malloc, per-lane kma_free
-    (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 1), # per-lane kma_malloc, collective free
-    (14, 1024, 32, 8192, 3, 1, 3, 8, 16, 3),  # per-lane malloc and free, 16 wavefronts per context
+    # variant bits 0x30: aggregation width 0 wavefront, 0x10/0x20/0x30
+    # work-group staging with 8/16/32-lane claim groups
+    (14, 1024, 4, 1024, 3, 1, 1, 64, 16, 0x20),     # one context per work-group
+    (14, 65536, 16, 4096, 4, 1, 1, 8, 16, 0x20),    # two work-groups per context
+    (12, 4096, 4, 1000, 20, 0, 1, 64, 64, 0x20),    # fixed 64 B, partial last work-group
+    (12, 4096, 4, 777, 10, 1, 5, 64, 64, 0x20),     # 4 KiB superblocks, mixed classes
+    (14, 64, 8, 2048, 3, 1, 9, 8, 16, 0x20),        # tiny heap: pool exhaustion, NULLs allowed
+    (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 0x22),    # collective malloc, per-lane kma_free
+    (14, 4096, 32, 8192, 4, 1, 4, 16, 32, 0x21),    # per-lane kma_malloc, collective free
+    (14, 1024, 32, 8192, 3, 1, 3, 8, 16, 3),        # per-lane malloc and free, 16 wavefronts per context
+    (14, 1024, 8, 2048, 3, 1, 6, 16, 16, 0x00),     # wavefront aggregation
+    (12, 4096, 4, 777, 10, 1, 7, 64, 64, 0x00),     # wavefront aggregation, partial last wavefront
+    (14, 64, 8, 2048, 3, 1, 8, 8, 16, 0x00),        # wavefront aggregation, pool exhaustion
+    (14, 1024, 8, 2048, 3, 1, 10, 16, 16, 0x10),    # 8-lane claim groups
+    (14, 1024, 8, 2048, 3, 1, 11, 16, 16, 0x30),    # 32-lane claim groups
+    (14, 64, 8, 2048, 3, 1, 12, 8, 16, 0x30),       # 32-lane claim groups, pool exhaustion
 ])
 def test_batched_invariants_in_simulation(sim, args):
     vals, waits, raw = _run(sim, *args)

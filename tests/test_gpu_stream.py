@@ -185,13 +185,14 @@ def test_stream_autotune_small():
     print("stream autotune", flush=True)
     with kma_amd.Stream(48, 4096, seed=9) as s:
         r = s.autotune(256 << 20, 14, null_limit=0, reps=1)
-        assert len(r["trials"]) == 36
+        assert len(r["trials"]) == 96
         assert all(t["violations"] == 0 for t in r["trials"]), r["trials"]
         best = max(r["trials"], key=lambda t: t["ops_per_s"])
         assert r["ops_per_s"] == best["ops_per_s"]
-        assert r["config"]["sb_log2"] == best["sb_log2"] and r["nwg"] == best["nwg"]
+        assert r["config"]["sb_log2"] == best["sb_log2"] and r["nwg"] == best["nwg"] and r["agg"] == best["agg"]
+        assert {t["agg"] for t in r["trials"]} == {0x00, 0x10, 0x20, 0x30}
         with kma_amd.Heap.from_config(r["cfg"]) as h:
-            st = s.run(h, form=kma_amd.FORM_BATCHED, nwg=r["nwg"], per_thread=r["per_thread"])
+            st = s.run(h, form=kma_amd.FORM_BATCHED, nwg=r["nwg"], per_thread=r["per_thread"], variant=r["agg"])
             check_run(st, s.info)
             h.trim()
             assert NatHeap(h).canonical_problems() == []
