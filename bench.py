@@ -40,12 +40,10 @@ NCTX = 4096
 SEED = 1
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 30  # the copy probe: reads and writes 1 GiB
-# Counter-measured HBM traffic of the batched launch, per round, from the
-# FETCH_SIZE and WRITE_SIZE profiles (scaled by the copy probe's reading).
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_full_occupancy_batched.md")
-
-
-L2_PROFILE = os.path.join(ROOT, "profiles", "l2_requests_full_occupancy_batched.md")
+# Counter-measured L2 requests and HBM traffic of the batched launch, per
+# round (FETCH_SIZE / WRITE_SIZE scaled by the copy probe's reading).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "counters_full_occupancy_batched.md")
+L2_PROFILE = TRAFFIC_PROFILE
 
 
 def l2_requests_per_round():
@@ -69,7 +67,7 @@ def l2_roofline(ms_per_round, store_peak):
     rate = req / (ms_per_round * 1e-3)
     return {"requests_per_round": req, "achieved_requests_per_s": rate,
             "peak_scattered_4b_stores_per_s": store_peak, "frac": rate / store_peak,
-            "source": "profiles/l2_requests_full_occupancy_batched.md"}
+            "source": "profiles/counters_full_occupancy_batched.md"}
 
 
 def traffic_per_round():
@@ -92,21 +90,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(rounds=4):
+def cpu_baseline(rounds=4, runs=5):
     """Reference algorithm, CPU restatement (oracle/refmodel.c, pthreads with
-    GCC atomics, same CAS protocol), on this host's cores."""
+    GCC atomics, same CAS protocol), one thread per host core (nproc), each
+    thread running a contiguous slice of the work-items as a CPU OpenCL
+    runtime runs a barrier-free kernel; the median of `runs` runs."""
     from oracle import refmodel as rm
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    r = rm.cpu_run(SB_LOG2, SBLOCKS, threads, NWI, rounds, 1, 0, SEED)
+    threads = max(1, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+    res = [rm.cpu_run(SB_LOG2, SBLOCKS, threads, NWI, rounds, 1, 0, SEED) for _ in range(runs)]
+    rates = sorted(r["ops"] / r["seconds"] for r in res)
+    med = rates[len(rates) // 2]
     return {
-        "value": r["ops"] / r["seconds"],
+        "value": med,
         "unit": "heap ops/s",
         "cores": threads,
         "kind": "port",
-        "sample": "rounds 0-%d of the full-occupancy trace, all %d work-items (%d ops, %.2f s); "
-                  "reference protocol incl. its kma.cl:338 write-after-release (%d read-back mismatches)"
-                  % (rounds - 1, NWI, r["ops"], r["seconds"], r["bad"]),
+        "sample": "median of %d runs (%s heap ops/s) of rounds 0-%d of the full-occupancy trace, all %d "
+                  "work-items (%d ops per run, %.1f s in all) on %d host threads (nproc); reference protocol "
+                  "incl. its kma.cl:338 write-after-release (%d read-back mismatches in all)"
+                  % (runs, ", ".join("%.3g" % x for x in rates), rounds - 1, NWI, res[0]["ops"],
+                     sum(r["seconds"] for r in res), threads, sum(r["bad"] for r in res)),
     }
 
 
@@ -250,9 +254,10 @@ def main():
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--only", choices=["plain", "fused", "batched"],
-                    help="run only this form of the full-occupancy workload, then the copy "
-                         "probe (the command the counter profiles are taken with)")
+    ap.add_argument("--only", choices=["plain", "fused", "batched", "stream"],
+                    help="run only this form of the full-occupancy workload, or (stream) the "
+                         "persistent form of the batched stream at the autotuner's usual pick, "
+                         "then the copy probe (the commands the counter profiles are taken with)")
     ap.add_argument("--serve-ms", type=float, default=10000.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -266,6 +271,18 @@ def main():
         raise SystemExit("no GPU: " + kma_amd.last_error())
 
     K, W = args.steps, args.warmup
+    if args.only == "stream":
+        with kma_amd.Stream(**STREAM) as s:
+            cfg = s.heap_config(STREAM_HEAP, 14, kma_amd.TABLE_POW2, 14)
+            with kma_amd.Heap.from_config(cfg) as h:
+                st = s.run(h, form=kma_amd.FORM_BATCHED, nwg=1024, per_thread=1, warmup=1, reps=K,
+                           variant=kma_amd.VAR_AGG_WG8)
+        gbps, cms = kma_amd.copy_probe(COPY_BYTES, warmup=1, reps=1)
+        print(json.dumps({"form": "stream persistent (8-lane work-group staging, 1,024 x 256, 1 slice per chunk)",
+                          "replays": K, "ms_per_replay": st["gpu_ms"], "kernel_ms_avg": st["kernel_ms_avg"],
+                          "ops": st["ops"], "nulls": st["nulls"], "bad": st["bad"], "copy_bytes": COPY_BYTES,
+                          "copy_gbps": gbps, "copy_ms": cms}), flush=True)
+        return
     if args.only:
         form = {"plain": kma_amd.FORM_PLAIN, "fused": kma_amd.FORM_FUSED, "batched": kma_amd.FORM_BATCHED}[args.only]
         with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, SB_LOG2, SBLOCKS, nctx=NCTX) as h:

@@ -48,14 +48,18 @@ static_assert(sizeof(gnode) == 32 && sizeof(glink) == 16, "graph record sizes");
 /* counters at stats[k], summed per wavefront */
 enum : uint32_t { G_EDGES, G_NODE_MALLOC, G_NODE_FREE, G_LINK_MALLOC, G_NULLS, G_BAD, G_N };
 
+/* Named fields, not an array: the compiler merges increments on different
+ * paths into one with a computed index, which would put an array in scratch. */
 struct gcount {
-	uint32_t v[G_N] = {};
+	uint32_t edges = 0, node_malloc = 0, node_free = 0, link_malloc = 0, nulls = 0, bad = 0;
 };
 
-__device__ void flush(unsigned long long *stats, const gcount &c)
+__device__ __forceinline__ void flush(unsigned long long *stats, const gcount &c)
 {
+	const uint32_t v[G_N] = {c.edges, c.node_malloc, c.node_free, c.link_malloc, c.nulls, c.bad};
+#pragma unroll
 	for (uint32_t k = 0; k < G_N; k++) {
-		uint64_t x = c.v[k];
+		uint64_t x = v[k];
 		for (int o = 32; o > 0; o >>= 1)
 			x += __shfl_xor(x, o);
 		if (__lane_id() == 0 && x)
@@ -87,7 +91,7 @@ __device__ __forceinline__ void adj_push(gnode *src, glink *l, gnode *sink)
  * insert race the winner's node is found on the next look-up and `cache`
  * stays with the caller.  NULL if a node was needed and cache was NULL
  * (the caller allocates and calls again). */
-__device__ __forceinline__ gnode *ensure_from(kma_tree *t, uint32_t key, gnode *&cache, gcount &c)
+__device__ __forceinline__ gnode *ensure_from(kma_tree *t, uint32_t key, gnode *&cache, uint32_t *lost)
 {
 	for (uint32_t attempt = 0; attempt < 4; attempt++) {
 		gnode *n = (gnode *)kma_tree_get(t, key);
@@ -103,7 +107,7 @@ __device__ __forceinline__ gnode *ensure_from(kma_tree *t, uint32_t key, gnode *
 			return n;
 		}
 	}
-	c.v[G_BAD]++; /* a lost race whose winner is not found */
+	*lost = 1; /* a lost race whose winner is not found */
 	return nullptr;
 }
 
@@ -111,44 +115,52 @@ __device__ __forceinline__ gnode *ensure_from(kma_tree *t, uint32_t key, gnode *
 struct lane_alloc {
 	kma_dheap *H;
 	kma_pma_dheap *P;
-	__device__ void *get(size_t size) { return P ? pma_malloc(P, size) : kma_malloc(H, size); }
-	__device__ void put(void *p)
+	__device__ __forceinline__ void *get(size_t size) { return P ? pma_malloc(P, size) : kma_malloc(H, size); }
+	__device__ __forceinline__ void put(void *p)
 	{
 		if (!P)
 			kma_free(H, p);
 	}
 };
 
-__device__ gnode *ensure_lane(kma_tree *t, uint32_t key, gnode *&cache, bool keep, lane_alloc &a, gcount &c)
+/* The node with `key` for a per-lane form, allocating (and, in the plain
+ * form, freeing a node that lost its insert race) as needed.  The counts are
+ * returned as values for the caller to add: increments of different fields
+ * on different paths were merged by the compiler into one increment through
+ * a selected address, which put the counters in scratch. */
+struct ensured {
+	gnode *n;
+	uint32_t mallocs, nulls, frees, bad;
+};
+
+__device__ __forceinline__ ensured ensure_lane(kma_tree *t, uint32_t key, gnode *&cache, bool keep, lane_alloc &a)
 {
+	uint32_t mallocs = 0, frees = 0;
 	for (uint32_t attempt = 0; attempt < 4; attempt++) {
 		gnode *n = (gnode *)kma_tree_get(t, key);
 		if (n)
-			return n;
+			return ensured{n, mallocs, 0u, frees, 0u};
 		if (!cache) {
 			cache = (gnode *)a.get(sizeof(gnode));
-			c.v[G_NODE_MALLOC]++;
-			if (!cache) {
-				c.v[G_NULLS]++;
-				return nullptr;
-			}
+			mallocs++;
+			if (!cache)
+				return ensured{nullptr, mallocs, 1u, frees, 0u};
 		}
 		st64(&cache->adj, 0);
 		cache->t.key = key;
 		if (kma_tree_add(t, &cache->t)) {
 			n = cache;
 			cache = nullptr;
-			return n;
+			return ensured{n, mallocs, 0u, frees, 0u};
 		}
 		if (!keep) {
 			/* plain form: the node that lost the insert race is freed */
 			a.put(cache);
-			c.v[G_NODE_FREE]++;
+			frees++;
 			cache = nullptr;
 		}
 	}
-	c.v[G_BAD]++; /* lost races whose winners were not found */
-	return nullptr;
+	return ensured{nullptr, mallocs, 0u, frees, 1u}; /* lost races whose winners were not found */
 }
 
 } // namespace
@@ -163,24 +175,33 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_lane(kma_dheap *H, kma_pma_dhe
 	gnode *cache = nullptr;
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-		gnode *src = ensure_lane(t, edges[2 * i], cache, keep != 0, a, c);
+		ensured e = ensure_lane(t, edges[2 * i], cache, keep != 0, a);
+		c.node_malloc += e.mallocs;
+		c.nulls += e.nulls;
+		c.node_free += e.frees;
+		c.bad += e.bad;
+		gnode *src = e.n;
 		if (!src)
 			break;
-		gnode *snk = ensure_lane(t, edges[2 * i + 1], cache, keep != 0, a, c);
+		e = ensure_lane(t, edges[2 * i + 1], cache, keep != 0, a);
+		c.node_malloc += e.mallocs;
+		c.nulls += e.nulls;
+		c.node_free += e.frees;
+		c.bad += e.bad;
+		gnode *snk = e.n;
 		if (!snk)
 			break;
 		glink *l = (glink *)a.get(sizeof(glink));
-		c.v[G_LINK_MALLOC]++;
-		if (!l) {
-			c.v[G_NULLS]++;
+		c.link_malloc++;
+		c.nulls += l ? 0u : 1u;
+		if (!l)
 			break;
-		}
 		adj_push(src, l, snk);
-		c.v[G_EDGES]++;
+		c.edges++;
 	}
 	if (cache) {
 		a.put(cache);
-		c.v[G_NODE_FREE] += P ? 0u : 1u;
+		c.node_free += P ? 0u : 1u;
 	}
 	flush(stats, c);
 }
@@ -204,37 +225,33 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_wg(kma_dheap *H, kma_tree *t, 
 		gnode *src = has ? (gnode *)kma_tree_get(t, ks) : nullptr;
 		bool need = has && !src && !cache;
 		void *p = kma_wg_malloc<GBLOCK, 16u>(H, need ? sizeof(gnode) : 0u, L);
-		if (need) {
+		if (need)
 			cache = (gnode *)p;
-			c.v[G_NODE_MALLOC]++;
-			c.v[G_NULLS] += p ? 0u : 1u;
-		}
+		c.node_malloc += need ? 1u : 0u;
+		c.nulls += need && !p ? 1u : 0u;
+		uint32_t lost = 0;
 		if (has && !src)
-			src = ensure_from(t, ks, cache, c);
+			src = ensure_from(t, ks, cache, &lost);
 		gnode *snk = has && src ? (gnode *)kma_tree_get(t, kk) : nullptr;
 		need = has && src && !snk && !cache;
 		p = kma_wg_malloc<GBLOCK, 16u>(H, need ? sizeof(gnode) : 0u, L);
-		if (need) {
+		if (need)
 			cache = (gnode *)p;
-			c.v[G_NODE_MALLOC]++;
-			c.v[G_NULLS] += p ? 0u : 1u;
-		}
+		c.node_malloc += need ? 1u : 0u;
+		c.nulls += need && !p ? 1u : 0u;
 		if (has && src && !snk)
-			snk = ensure_from(t, kk, cache, c);
+			snk = ensure_from(t, kk, cache, &lost);
+		c.bad += lost;
 		const bool linkit = has && src && snk;
 		glink *l = (glink *)kma_wg_malloc<GBLOCK, 16u>(H, linkit ? sizeof(glink) : 0u, L);
-		if (linkit) {
-			c.v[G_LINK_MALLOC]++;
-			if (l) {
-				adj_push(src, l, snk);
-				c.v[G_EDGES]++;
-			} else {
-				c.v[G_NULLS]++;
-			}
-		}
+		if (linkit && l)
+			adj_push(src, l, snk);
+		c.link_malloc += linkit ? 1u : 0u;
+		c.edges += linkit && l ? 1u : 0u;
+		c.nulls += linkit && !l ? 1u : 0u;
 		dead = dead || (i < n && !(linkit && l));
 	}
-	c.v[G_NODE_FREE] += cache ? 1u : 0u;
+	c.node_free += cache ? 1u : 0u;
 	kma_wg_free<GBLOCK, 16u>(H, cache, L);
 	flush(stats, c);
 }

@@ -42,8 +42,10 @@ constexpr uint64_t OFF_MASK = (1ull << 40) - 1ull;
 /* a free gives up waiting for its malloc's result after 2 s of wall clock */
 constexpr uint64_t RESULT_WAIT_TICKS = 200000000ull;
 
+/* Named fields, not an array: the compiler merges increments on different
+ * paths into one with a computed index, which would put an array in scratch. */
 struct counters {
-	uint32_t v[C_N];
+	uint32_t ops = 0, mok = 0, nul = 0, fok = 0, fnull = 0, bad = 0, tmo = 0, large = 0, lnull = 0;
 };
 
 __device__ __forceinline__ uint32_t pattern(uint64_t j)
@@ -75,8 +77,10 @@ struct phases {
 /* all 64 lanes of the wavefront are active when this runs */
 __device__ __forceinline__ void flush(uint64_t *ctl, const counters &c, const phases &ph)
 {
+	const uint32_t v[C_N] = {c.ops, c.mok, c.nul, c.fok, c.fnull, c.bad, c.tmo, c.large, c.lnull};
+#pragma unroll
 	for (int k = 0; k < C_N; k++) {
-		uint64_t x = c.v[k];
+		uint64_t x = v[k];
 		for (int o = 32; o > 0; o >>= 1)
 			x += __shfl_xor(x, o);
 		if (__lane_id() == 0 && x)
@@ -92,17 +96,21 @@ __device__ __forceinline__ void flush(uint64_t *ctl, const counters &c, const ph
 __device__ __forceinline__ void count_malloc(const kma_dheap *H, uint32_t size, const void *p, counters &c)
 {
 	bool large = size > H->max_small;
-	/* constant indices only: a computed index would put c in scratch */
-	c.v[C_MOK] += p != nullptr;
-	c.v[C_NULL] += p == nullptr;
-	c.v[C_LARGE] += large;
-	c.v[C_LNULL] += large && !p;
+	c.mok += p != nullptr;
+	c.nul += p == nullptr;
+	c.large += large;
+	c.lnull += large && !p;
 }
 
+/* Outcome of a free's wait for its malloc's result. */
+enum { T_OK, T_BAD, T_NULL, T_TIMEOUT };
+
 /* The block of request t's malloc: waits for the result slot to carry this
- * run's epoch, checks the pattern.  NULL if the malloc failed (or the wait
- * gave up, which is counted). */
-__device__ __forceinline__ void *take_target(const kma_dheap *H, uint64_t *res, uint32_t t, uint32_t ep, counters &c)
+ * run's epoch and checks the pattern.  NULL if the malloc failed or the wait
+ * gave up; *kind says which (counted by the caller with arithmetic only: an
+ * increment per branch here was merged by the compiler into one through a
+ * selected address, which put the counters in scratch). */
+__device__ __forceinline__ void *take_target(const kma_dheap *H, uint64_t *res, uint32_t t, uint32_t ep, uint32_t *kind)
 {
 	using namespace kma_nat;
 	uint64_t v = ld(res + t);
@@ -117,21 +125,27 @@ __device__ __forceinline__ void *take_target(const kma_dheap *H, uint64_t *res, 
 			if ((v >> 40) == ep)
 				break;
 			if (wall_clock64() - t0 > RESULT_WAIT_TICKS) {
-				c.v[C_TMO]++;
+				*kind = T_TIMEOUT;
 				return nullptr;
 			}
 		}
 	}
 	uint64_t off = v & OFF_MASK;
 	if (!off) {
-		c.v[C_FNULL]++;
+		*kind = T_NULL;
 		return nullptr;
 	}
 	char *p = base(H) + off;
-	if (ld((uint32_t *)p) != pattern(t))
-		c.v[C_BAD]++;
-	c.v[C_FOK]++;
+	*kind = ld((uint32_t *)p) != pattern(t) ? T_BAD : T_OK;
 	return p;
+}
+
+__device__ __forceinline__ void count_free(uint32_t kind, counters &c)
+{
+	c.fok += kind == T_OK || kind == T_BAD;
+	c.bad += kind == T_BAD;
+	c.fnull += kind == T_NULL;
+	c.tmo += kind == T_TIMEOUT;
 }
 
 __device__ __forceinline__ uint64_t result_word(const kma_dheap *H, const void *p, uint32_t ep)
@@ -184,7 +198,9 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_plain(kma_dheap *H, const uin
 	ph.mark(0);
 	void *q = nullptr;
 	if (isf) {
-		q = take_target(H, res, r & 0x7fffffffu, ep, c);
+		uint32_t kind;
+		q = take_target(H, res, r & 0x7fffffffu, ep, &kind);
+		count_free(kind, c);
 		if (log)
 			ts[j] = wall_clock64();
 	}
@@ -193,7 +209,7 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_plain(kma_dheap *H, const uin
 		kma_free(H, q);
 	ph.mark(2);
 	ph.n = 1;
-	c.v[C_OPS] = in;
+	c.ops = in;
 	flush(ctl, c, ph);
 }
 
@@ -241,7 +257,7 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, cons
 				if (log)
 					ts[j] = wall_clock64();
 			}
-			c.v[C_OPS] += in;
+			c.ops += in;
 		}
 		ph.mark(0);
 		for (uint32_t m = 0; m < nsl; m++) {
@@ -250,7 +266,9 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, cons
 			const uint32_t r = in ? req[j] : 0u;
 			void *q = nullptr;
 			if (in && (r >> 31)) {
-				q = take_target(H, res, r & 0x7fffffffu, ep, c);
+				uint32_t kind;
+				q = take_target(H, res, r & 0x7fffffffu, ep, &kind);
+				count_free(kind, c);
 				if (log)
 					ts[j] = wall_clock64();
 			}
