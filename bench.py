@@ -183,7 +183,7 @@ def bench_stream(reps):
 # Serving loop: Poisson arrivals (2,000 jobs/ms), log-normal lengths
 # (mu 6, sigma 1 steps, capped at 8,192), one 16 KiB block per live job per
 # step, whole chain freed at completion, 64 GiB heap (1 MiB superblocks of
-# 64 blocks), admission below 90 % use, 10 s, seed 4.
+# 64 blocks), admission within 90 % of the heap, 10 s, seed 4.
 SERVE = dict(rate_per_ms=2000.0, len_mu=6.0, len_sigma=1.0, duration_ms=10000.0, max_len=8192,
              block_bytes=16384, watermark_pct=90, max_live=32768, seed=4)
 SERVE_HEAP_SB = 65536  # x 1 MiB = 64 GiB
@@ -217,30 +217,42 @@ def bench_graph():
     return out
 
 
-def bench_serve(duration_ms):
+def bench_serve(duration_ms, admit):
     import kma_amd
 
-    spec = kma_amd.serve_spec(**dict(SERVE, duration_ms=duration_ms))
+    spec = kma_amd.serve_spec(**dict(SERVE, duration_ms=duration_ms, admit=admit))
     with kma_amd.Heap(layout=kma_amd.LAYOUT_NATIVE, sb_log2=20, sblocks=SERVE_HEAP_SB, classes=[16384]) as h:
         st, _ = kma_amd.serve_run(h, spec)
-    log("bench: serving loop %s" % {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
+    policy = "projected" if admit == kma_amd.ADMIT_PROJECTED else "watermark"
+    log("bench: serving loop, %s admission %s" % (policy, {k: (round(v, 3) if isinstance(v, float) else v)
+                                                            for k, v in st.items()}))
     return {
         "workload": "Poisson arrivals 2,000 jobs/ms, log-normal lengths (mu 6, sigma 1) capped at 8,192 steps, "
-                    "+16 KiB per live job per step, chain freed at completion, 64 GiB heap, admission below "
-                    "90%% use, %g ms, seed 4" % duration_ms,
+                    "+16 KiB per live job per step, chain freed at completion, 64 GiB heap, admission within "
+                    "90%% of the heap, %g ms, seed 4" % duration_ms,
+        "admission": policy,
         "ops_per_s": st["ops_per_s"],
+        "goodput_jobs_per_s": st["goodput_per_s"],
         "run_ms": st["run_ms"],
         "steps": st["steps"],
         "jobs_arrived": st["jobs"],
         "jobs_admitted": st["admitted"],
         "jobs_done": st["done"],
+        "jobs_preempted": st["preempted"],
+        "preempted_over_admitted": st["preempt_frac"],
+        "mean_live_jobs": st["mean_live"],
         "mallocs": st["mallocs"],
         "frees": st["frees"],
         "stalls": st["stalls"],
         "malloc_latency_ns": {"p50": st["lat_p50_ns"], "p99": st["lat_p99_ns"], "max": st["lat_max_ns"]},
         "admission_wait_us": {"p50": st["wait_p50_us"], "p99": st["wait_p99_us"]},
+        "phase_us_per_step": st["phase_us"],
+        "busy_frac": st["busy_frac"],
         "peak_used_frac": st["peak_used_frac"],
-        "violations": int(st["bad"] + st["aborted"]),
+        "peak_projected_frac": st["peak_proj_frac"],
+        "projection_checked_steps": st["proj_checked"],
+        "projection_mismatches": st["proj_mismatch"],
+        "violations": int(st["bad"] + st["aborted"] + st["proj_mismatch"]),
         "work_groups": st["nwg"],
     }
 
@@ -254,10 +266,11 @@ def main():
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--only", choices=["plain", "fused", "batched", "stream"],
+    ap.add_argument("--only", choices=["plain", "fused", "batched", "stream", "serve"],
                     help="run only this form of the full-occupancy workload, or (stream) the "
                          "persistent form of the batched stream at the autotuner's usual pick, "
-                         "then the copy probe (the commands the counter profiles are taken with)")
+                         "then the copy probe (the commands the counter profiles are taken with); "
+                         "(serve) the serving loop with projected admission")
     ap.add_argument("--serve-ms", type=float, default=10000.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -271,6 +284,9 @@ def main():
         raise SystemExit("no GPU: " + kma_amd.last_error())
 
     K, W = args.steps, args.warmup
+    if args.only == "serve":
+        print(json.dumps(bench_serve(args.serve_ms, kma_amd.ADMIT_PROJECTED)), flush=True)
+        return
     if args.only == "stream":
         with kma_amd.Stream(**STREAM) as s:
             cfg = s.heap_config(STREAM_HEAP, 14, kma_amd.TABLE_POW2, 14)
@@ -379,7 +395,8 @@ def main():
     if not args.no_stream:
         out["batched_stream"] = bench_stream(5)
     if not args.no_serve:
-        out["serving"] = bench_serve(args.serve_ms)
+        out["serving"] = bench_serve(args.serve_ms, kma_amd.ADMIT_PROJECTED)
+        out["serving_watermark_admission"] = bench_serve(args.serve_ms, kma_amd.ADMIT_WATERMARK)
     if not args.no_graph:
         out["graph_build"] = bench_graph()
     if not args.no_cpu:

@@ -19,7 +19,9 @@
  *   bits 32..36  class + 1 (0: superblock is free / in the pool)
  *   bit  37      owned: attached to a context head
  *   bits 38..40  home XCD: the XCD whose context attached it
- *   bits 41..63  generation, bumped each time the superblock returns to the pool
+ *   bit  41      listed: detached, partly free, in the partial index of (home, class)
+ *   bits 42..63  generation, bumped each time the superblock returns to the pool
+ *                or is attached again from the partial index
  * A wavefront group of n same-class lanes reserves with ONE atomicAdd(n) on
  * this word; the returned old value says (atomically) whether the superblock
  * is still the incarnation the group read from the context head (generation),
@@ -96,6 +98,9 @@ typedef struct kma_dheap {
 	uint64_t off_spare;    /* context spare lines, 128 B per context */
 	uint32_t spare_slots;  /* slots used per spare line (0..16) */
 	uint32_t pad1;
+	uint64_t off_pidx;     /* partial index: [8 XCDs][nclasses] blocks of pidx_words u64 */
+	uint32_t pidx_words;   /* u64 per (XCD, class) block: levels 0, 1, 2 (see pidx) */
+	uint32_t pad2;
 } kma_dheap;
 
 /* Per-heap counters (u64 at off_stats). */
@@ -108,7 +113,9 @@ typedef struct kma_dheap {
 #define KMA_STAT_WAIT_POOL 6    /* a pool push/pop hit the watchdog */
 #define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
 #define KMA_STAT_MIGRATED 8     /* superblocks moved from dirty to clean shards in a kernel */
-#define KMA_NSTATS 9
+#define KMA_STAT_LISTED 9       /* detached superblocks put into a partial index */
+#define KMA_STAT_REATTACHED 10  /* listed superblocks attached to a head again */
+#define KMA_NSTATS 11
 /* The stats block is 256 B of counters and one 128-B line per XCD x holding
  * x's shard mask (bit k set when shard x*S + k may hold a superblock, see
  * pool_push / pool_pop_shard) and, in the next word, x's clean request
@@ -133,8 +140,9 @@ static constexpr uint64_t ST_CLASS_SHIFT = 32;
 static constexpr uint64_t ST_CLASS = 0x1full << ST_CLASS_SHIFT;
 static constexpr uint64_t ST_OWNED = 1ull << 37;
 static constexpr uint64_t ST_HOME_SHIFT = 38;
-static constexpr uint64_t ST_GEN_SHIFT = 41;
-static constexpr uint32_t ST_GEN_MASK = 0x7fffffu;
+static constexpr uint64_t ST_LISTED = 1ull << 41;
+static constexpr uint64_t ST_GEN_SHIFT = 42;
+static constexpr uint32_t ST_GEN_MASK = 0x3fffffu;
 
 /* Watchdog for retry loops: reads the wall clock every 256 retries only. */
 struct waiter {
@@ -256,6 +264,9 @@ __device__ __forceinline__ uint64_t *creq(const kma_dheap *H, uint32_t x) { retu
 /* wall-clock tick until which XCD x's pops skip the search beyond its own
  * spares and shards (set by a search that found nothing) */
 __device__ __forceinline__ uint64_t *exhausted(const kma_dheap *H, uint32_t x) { return pmask(H, x) + 2; }
+/* non-zero from a pool search of XCD x that found nothing to the next pop
+ * of x that finds a superblock: x's free capacity is in partly free ones */
+__device__ __forceinline__ uint64_t *pressure(const kma_dheap *H, uint32_t x) { return pmask(H, x) + 3; }
 __device__ __forceinline__ uint64_t psum_words(const kma_dheap *H) { return (((H->npages + 63) >> 6) + 63) >> 6; }
 __device__ __forceinline__ uint64_t *psum(const kma_dheap *H, uint32_t l) { return (uint64_t *)(base(H) + H->off_psum) + l * psum_words(H); }
 __device__ __forceinline__ uint64_t psum2_words(const kma_dheap *H) { return (psum_words(H) + 63) >> 6; }
@@ -580,7 +591,7 @@ __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint3
 #ifndef KMA_EXHAUSTED_TICKS
 #define KMA_EXHAUSTED_TICKS 5000u
 #endif
-__device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t salt, uint32_t *out)
+__device__ inline bool pool_search(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t salt, uint32_t *out)
 {
 	uint32_t S = shards_per_xcd(H), D = dirty_per_xcd(H);
 	uint64_t all = S == 64 ? ~0ull : ((1ull << S) - 1ull);
@@ -653,8 +664,152 @@ __device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, ui
 	return false;
 }
 
-/* Called by whoever produced state `s` == {count 0, not owned, class set},
- * running on XCD x. */
+/* pool_search, keeping XCD x's pressure word: set by a search that found
+ * nothing, cleared by the next one that found a superblock (read with the
+ * search's own first loads, so a pop under no pressure adds no round trip) */
+__device__ inline bool pool_pop(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t salt, uint32_t *out)
+{
+	const uint64_t pr = ld(pressure(H, x));
+	const bool ok = pool_search(H, x, ctx, salt, out);
+	if (ok && pr)
+		st(pressure(H, x), (uint64_t)0);
+	else if (!ok && !pr)
+		st(pressure(H, x), (uint64_t)1);
+	return ok;
+}
+
+/* ---- partial index ----
+ * A superblock detached from its head (full when it was detached) whose
+ * frees leave at least an eighth of it free is "listed": the LISTED bit of
+ * its state word is set and its bit is set in the partial index of (home
+ * XCD, class).  A head refill whose pool search finds no empty superblock
+ * takes one from its own XCD's index of its class and attaches it again
+ * (generation bumped), so capacity freed block by block is reused before
+ * the superblock drains (a detached superblock is otherwise reachable only
+ * once it is empty).  The index of one (XCD, class) is three levels of u64
+ * words: level 0 has one bit per superblock, level 1 one bit per level-0
+ * word ("may be non-zero"), level 2 one bit per level-1 word.  A put sets
+ * the superblock's bit and, if that word was zero, the word's bit one level
+ * up (and so on); whoever leaves a word zero (or finds it zero) clears its
+ * bit one level up, re-reads the word and sets the bit again if a put came
+ * in -- the protocol of the pool's shard masks and the page summaries, so a
+ * set bit is a hint and a non-zero word always has its bit set once puts
+ * have finished.  A take walks down set bits (from a salted position) and
+ * claims a superblock with one atomicAnd on its level-0 word; the free that
+ * empties a listed superblock clears its bit the same way, and whichever of
+ * the two clears it owns the superblock.  Invariants: LISTED is set only
+ * while the superblock is not owned and has a class, and then its bit is
+ * set or being set (by its lister); whoever clears the bit clears LISTED.
+ * Indexes are per XCD for the same reason as dirty shards: a block is
+ * reused on the XCD whose L2 may hold its lines. */
+__device__ __forceinline__ uint32_t pidx_w0(const kma_dheap *H) { return (uint32_t)((H->nsb + 63u) >> 6); }
+__device__ __forceinline__ uint32_t pidx_w1(const kma_dheap *H) { return (pidx_w0(H) + 63u) >> 6; }
+__device__ __forceinline__ uint64_t *pidx(const kma_dheap *H, uint32_t x, uint32_t c)
+{
+	return (uint64_t *)(base(H) + H->off_pidx) + ((uint64_t)x * H->nclasses + c) * H->pidx_words;
+}
+/* First word of level lv (0, 1, 2) in an index block. */
+__device__ __forceinline__ uint32_t pidx_level(const kma_dheap *H, uint32_t lv)
+{
+	return lv == 0u ? 0u : lv == 1u ? pidx_w0(H) : pidx_w0(H) + pidx_w1(H);
+}
+/* Free slots a detached superblock needs before it is listed. */
+__device__ __forceinline__ uint32_t part_room(uint32_t cap) { return cap >= 8u ? cap / 8u : 1u; }
+
+/* Set bit i of level lv; a word that was zero sets its own bit one level up. */
+__device__ inline void pidx_set(const kma_dheap *H, uint64_t *P, uint32_t lv, uint32_t i)
+{
+	for (; lv < 3u; lv++, i >>= 6)
+		if (f_or(P + pidx_level(H, lv) + (i >> 6), (uint64_t)1 << (i & 63u)) != 0)
+			return;
+}
+
+/* Word i of level lv (0 or 1) was seen zero (after the caller's RMW, or by a
+ * load): clear its bit one level up, re-read it and set the bit again if a
+ * put came in; a level-1 word left zero by that clear continues one level
+ * up. */
+__device__ inline void pidx_emptied(const kma_dheap *H, uint64_t *P, uint32_t lv, uint32_t i)
+{
+	for (; lv < 2u; lv++, i >>= 6) {
+		uint64_t *up = P + pidx_level(H, lv + 1u) + (i >> 6);
+		const uint64_t b = (uint64_t)1 << (i & 63u);
+		const uint64_t o = f_and(up, ~b);
+		if (ld(P + pidx_level(H, lv) + i) != 0) {
+			pidx_set(H, P, lv + 1u, i);
+			return;
+		}
+		if ((o & ~b) != 0)
+			return;
+	}
+}
+
+/* Lowest set bit of m after rotating it right by r. */
+__device__ __forceinline__ uint32_t pick_bit(uint64_t m, uint32_t r)
+{
+	r &= 63u;
+	const uint64_t rot = r ? (m >> r) | (m << (64u - r)) : m;
+	return (uint32_t)(__ffsll((unsigned long long)rot) - 1 + (int)r) & 63u;
+}
+
+__device__ __forceinline__ void pidx_put(const kma_dheap *H, uint32_t x, uint32_t c, uint32_t sb)
+{
+	pidx_set(H, pidx(H, x, c), 0u, sb);
+}
+
+/* Take any listed superblock of (x, c) out of the index; false if none. */
+__device__ inline bool pidx_take(const kma_dheap *H, uint32_t x, uint32_t c, uint32_t salt, uint32_t *out)
+{
+	uint64_t *P = pidx(H, x, c);
+	const uint32_t w1 = pidx_w1(H), w2 = (w1 + 63u) >> 6;
+	const uint32_t L1 = pidx_level(H, 1u), L2 = pidx_level(H, 2u);
+	for (uint32_t a = 0; a < w2; a++) {
+		const uint32_t i2 = (a + salt) % w2;
+		for (uint64_t t2 = ld(P + L2 + i2); t2;) {
+			const uint32_t j2 = pick_bit(t2, salt);
+			t2 &= ~((uint64_t)1 << j2);
+			const uint32_t i1 = i2 * 64u + j2;
+			uint64_t t1 = ld(P + L1 + i1);
+			if (!t1)
+				pidx_emptied(H, P, 1u, i1);
+			while (t1) {
+				const uint32_t j1 = pick_bit(t1, salt >> 6);
+				t1 &= ~((uint64_t)1 << j1);
+				const uint32_t i0 = i1 * 64u + j1;
+				uint64_t t0 = ld(P + i0);
+				while (t0) {
+					const uint64_t b = (uint64_t)1 << pick_bit(t0, salt >> 12);
+					const uint64_t o = f_and(P + i0, ~b);
+					if (o & b) {
+						if ((o & ~b) == 0)
+							pidx_emptied(H, P, 0u, i0);
+						*out = i0 * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
+						return true;
+					}
+					t0 = o & ~b;
+				}
+				pidx_emptied(H, P, 0u, i0);
+			}
+		}
+	}
+	return false;
+}
+
+/* Clear sb's own bit in (x, c)'s index; false if it was clear (a refill
+ * took it, or its lister has not set it yet). */
+__device__ inline bool pidx_remove(const kma_dheap *H, uint32_t x, uint32_t c, uint32_t sb)
+{
+	uint64_t *P = pidx(H, x, c);
+	const uint64_t b = (uint64_t)1 << (sb & 63u);
+	const uint64_t o = f_and(P + (sb >> 6), ~b);
+	if (!(o & b))
+		return false;
+	if ((o & ~b) == 0)
+		pidx_emptied(H, P, 0u, sb >> 6);
+	return true;
+}
+
+/* Called by whoever produced state `s` == {count 0, not owned, not listed,
+ * class set}, running on XCD x. */
 __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t x, uint32_t ctx)
 {
 	uint32_t g = st_gen(s);
@@ -670,16 +825,162 @@ __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, 
 			return;
 		}
 		s = seen;
-		if (st_count(s) != 0 || (s & ST_OWNED) || st_cls1(s) == 0 || st_gen(s) != g)
+		if (st_count(s) != 0 || (s & (ST_OWNED | ST_LISTED)) || st_cls1(s) == 0 || st_gen(s) != g)
 			return;
 	}
+}
+
+/* XCD x's last pool search found no empty superblock (its refills are
+ * taking listed superblocks). */
+__device__ __forceinline__ bool under_pressure(const kma_dheap *H, uint32_t x)
+{
+	return ld(pressure(H, x)) != 0;
+}
+
+/* List detached superblock sb (state s as last seen: not owned, class set,
+ * count in 1..lim): set LISTED with a CAS while that still holds, then set
+ * its bit in the index of (home, class).  A free that empties it before the
+ * bit is set finds no bit and leaves it to this call: the state is read
+ * again after the put, and if this incarnation (same generation, still
+ * listed, not owned) has emptied, its bit is taken back here and it is
+ * released (a refill, or that free, may have taken the bit first: then
+ * nothing is left to do). */
+__device__ inline void try_list(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t lim, uint32_t x, uint32_t ctx)
+{
+	const uint32_t c1 = st_cls1(s);
+	for (;;) {
+		if ((s & (ST_OWNED | ST_LISTED)) || st_cls1(s) != c1 || st_count(s) == 0 || st_count(s) > lim)
+			return;
+		const uint64_t seen = cas(state(H, sb), s, s | ST_LISTED);
+		if (seen == s)
+			break;
+		s = seen;
+	}
+	s |= ST_LISTED;
+	pidx_put(H, st_home(s), c1 - 1u, sb);
+	bump_stat(H, KMA_STAT_LISTED);
+	const uint64_t now = ld(state(H, sb));
+	if (st_count(now) == 0 && (now & ~ST_COUNT) == (s & ~ST_COUNT) &&
+	    pidx_remove(H, st_home(s), c1 - 1u, sb)) {
+		const uint64_t r = f_and(state(H, sb), ~ST_LISTED) & ~ST_LISTED;
+		if (st_count(r) == 0 && !(r & ST_OWNED))
+			try_release(H, sb, r, x, ctx);
+	}
+}
+
+/* The free that emptied listed superblock sb (state s: count 0, listed):
+ * clear its index bit and release it; if the bit was clear, a refill took it
+ * (and attaches it) or its lister has not set it yet (the lister then finds
+ * it empty and releases it, see try_list). */
+__device__ inline void unlist_release(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t x, uint32_t ctx)
+{
+	if (!pidx_remove(H, st_home(s), st_cls1(s) - 1u, sb))
+		return;
+	/* out of the index: only a reservation still in flight against the head
+	 * it once was can change the count now, and then its free releases it */
+	s = f_and(state(H, sb), ~ST_LISTED) & ~ST_LISTED;
+	if (st_count(s) == 0 && !(s & ST_OWNED))
+		try_release(H, sb, s, x, ctx);
+}
+
+/* After a count decrement of m on superblock sb left state ns: release it
+ * if it emptied with no owner; list it, while detached, when it crosses to
+ * an eighth free, or at its first free while its home XCD's pool is
+ * exhausted (the free capacity of a nearly full heap is a few blocks per
+ * superblock, all of which must stay reachable). */
+__device__ __forceinline__ void after_sub(const kma_dheap *H, uint32_t sb, uint64_t ns, uint32_t m, uint32_t x,
+					 uint32_t ctx)
+{
+	if ((ns & ST_OWNED) || st_cls1(ns) == 0)
+		return;
+	const uint32_t cnt = st_count(ns);
+	if (cnt == 0) {
+		if (ns & ST_LISTED)
+			unlist_release(H, sb, ns, x, ctx);
+		else
+			try_release(H, sb, ns, x, ctx);
+		return;
+	}
+	if (ns & ST_LISTED)
+		return;
+	const uint32_t cap = H->class_cap[st_cls1(ns) - 1u], lim = cap - part_room(cap);
+	if (cnt <= lim && cnt + m > lim)
+		try_list(H, sb, ns, lim, x, ctx);
+	else if (cnt > lim && cnt + m >= cap && under_pressure(H, st_home(ns)))
+		try_list(H, sb, ns, cap - 1u, x, ctx);
+}
+
+/* After the owned bit was cleared (state s): release or list. */
+__device__ __forceinline__ void after_detach(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t x, uint32_t ctx)
+{
+	if (st_cls1(s) == 0)
+		return;
+	const uint32_t cap = H->class_cap[st_cls1(s) - 1u], cnt = st_count(s);
+	if (cnt == 0)
+		try_release(H, sb, s, x, ctx);
+	else if (cnt <= cap - part_room(cap))
+		try_list(H, sb, s, cap - part_room(cap), x, ctx);
+	else if (cnt < cap && under_pressure(H, st_home(s)))
+		try_list(H, sb, s, cap - 1u, x, ctx);
 }
 
 __device__ __forceinline__ void undo(const kma_dheap *H, uint32_t sb, uint32_t m, uint32_t x, uint32_t ctx)
 {
 	uint64_t s = fsub(state(H, sb), (uint64_t)m) - m;
-	if (st_count(s) == 0 && !(s & ST_OWNED) && st_cls1(s) != 0)
-		try_release(H, sb, s, x, ctx);
+	after_sub(H, sb, s, m, x, ctx);
+}
+
+/* Quiescent heap only (kma_trim): release every listed superblock of
+ * level-0 word i0 of (x, c)'s index that holds no live block (the summary
+ * levels are rebuilt afterwards by pidx_rebuild). */
+__device__ inline void trim_pidx_word(const kma_dheap *H, uint32_t x, uint32_t c, uint32_t i0)
+{
+	uint64_t *w = pidx(H, x, c) + i0;
+	for (uint64_t m = ld(w); m; m &= m - 1u) {
+		const uint32_t sb = i0 * 64u + (uint32_t)(__ffsll((unsigned long long)m) - 1);
+		const uint64_t s = ld(state(H, sb));
+		if (st_count(s) != 0 || !(s & ST_LISTED))
+			continue;
+		f_and(w, ~((uint64_t)1 << (sb & 63u)));
+		try_release(H, sb, f_and(state(H, sb), ~ST_LISTED) & ~ST_LISTED, x, x * (H->nctx / KMA_NXCD));
+	}
+}
+
+/* Quiescent heap only: summary levels 1 and 2 of (x, c)'s index exactly
+ * from level 0. */
+__device__ inline void pidx_rebuild(const kma_dheap *H, uint32_t x, uint32_t c)
+{
+	uint64_t *P = pidx(H, x, c);
+	const uint32_t w0 = pidx_w0(H), w1 = pidx_w1(H), w2 = (w1 + 63u) >> 6;
+	for (uint32_t i1 = 0; i1 < w1; i1++) {
+		uint64_t v = 0;
+		for (uint32_t j = 0; j < 64u && i1 * 64u + j < w0; j++)
+			v |= (uint64_t)(ld(P + i1 * 64u + j) != 0) << j;
+		st(P + w0 + i1, v);
+	}
+	for (uint32_t i2 = 0; i2 < w2; i2++) {
+		uint64_t v = 0;
+		for (uint32_t j = 0; j < 64u && i2 * 64u + j < w1; j++)
+			v |= (uint64_t)(ld(P + w0 + i2 * 64u + j) != 0) << j;
+		st(P + w0 + w1 + i2, v);
+	}
+}
+
+/* Attach listed superblock sb (just taken out of the index) to a head of
+ * class c: owned, not listed, count + n, generation bumped (reservations
+ * still in flight against the head it left then see a new incarnation).
+ * Returns the state before. */
+__device__ __forceinline__ uint64_t attach_listed(const kma_dheap *H, uint32_t sb, uint32_t n)
+{
+	uint64_t s = ld(state(H, sb));
+	for (;;) {
+		const uint64_t g = (uint64_t)((st_gen(s) + 1u) & ST_GEN_MASK) << ST_GEN_SHIFT;
+		const uint64_t nv = (((s & ~(ST_LISTED | (~0ull << ST_GEN_SHIFT))) | ST_OWNED) + n) | g;
+		const uint64_t seen = cas(state(H, sb), s, nv);
+		if (seen == s)
+			return s;
+		s = seen;
+	}
 }
 
 struct resv {
@@ -714,20 +1015,28 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 				h = seen;
 				continue;
 			}
-			uint32_t sb;
-			if (!pool_pop(H, x, ctx, 0u, &sb)) {
+			/* an empty superblock from the pool search, else a listed
+			 * partly free one of this XCD and class */
+			uint32_t sb, gen;
+			uint64_t old;
+			if (pool_pop(H, x, ctx, 0u, &sb)) {
+				old = fadd(state(H, sb), ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ST_OWNED |
+								 ((uint64_t)x << ST_HOME_SHIFT) | n);
+				gen = st_gen(old);
+			} else if (pidx_take(H, x, c, ctx, &sb)) {
+				old = attach_listed(H, sb, n);
+				gen = (st_gen(old) + 1u) & ST_GEN_MASK;
+				bump_stat(H, KMA_STAT_REATTACHED);
+			} else {
 				st(hp, HEAD_EMPTY);
 				bump_stat(H, KMA_STAT_POOL_EMPTY);
 				return resv{0, 0, 0, 0};
 			}
-			uint64_t add = ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ST_OWNED |
-				       ((uint64_t)x << ST_HOME_SHIFT) | n;
-			uint64_t old = fadd(state(H, sb), add);
 			uint32_t cnt = st_count(old);
 			uint32_t got = min(n, sat_sub(cap, cnt));
 			if (got < n)
 				fsub(state(H, sb), (uint64_t)(n - got));
-			h = ((uint64_t)st_gen(old) << 32) | (uint64_t)(sb + 2u);
+			h = ((uint64_t)gen << 32) | (uint64_t)(sb + 2u);
 			st(hp, h);
 			if (got == 0)
 				continue;
@@ -752,9 +1061,7 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 			undo(H, sb, n - got, x, ctx);
 			uint64_t seen = cas(hp, h, HEAD_EMPTY);
 			if (seen == h) {
-				uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
-				if (st_count(s) == 0 && st_cls1(s) != 0)
-					try_release(H, sb, s, x, ctx);
+				after_detach(H, sb, f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED, x, ctx);
 				seen = HEAD_EMPTY;
 			}
 			if (got == 0) {
@@ -1417,9 +1724,7 @@ __device__ __forceinline__ void kma_free(kma_dheap *H, void *ptr)
 	uint32_t ctx = ctx_of(H, x);
 	if (live && l == (uint32_t)__ffsll((unsigned long long)(grp & lv)) - 1u) {
 		uint32_t n = (uint32_t)__popcll(grp & lv);
-		uint64_t ns = fsub(state(H, sb), (uint64_t)n) - n;
-		if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
-			try_release(H, sb, ns, x, ctx);
+		after_sub(H, sb, fsub(state(H, sb), (uint64_t)n) - n, n, x, ctx);
 	}
 }
 

@@ -420,11 +420,8 @@ __device__ __forceinline__ void kma_wg_free(kma_dheap *H, void *ptr, kma_wg_lds<
 		uint32_t dbl = L.hn[slot] - n;
 		if (dbl)
 			fadd(stats(H) + KMA_STAT_DOUBLE_FREE, (uint64_t)dbl);
-		if (n) {
-			uint64_t ns = fsub(state(H, sb), (uint64_t)n) - n;
-			if (st_count(ns) == 0 && !(ns & ST_OWNED) && st_cls1(ns) != 0)
-				try_release(H, sb, ns, x, ctx);
-		}
+		if (n)
+			after_sub(H, sb, fsub(state(H, sb), (uint64_t)n) - n, n, x, ctx);
 	}
 	__syncthreads();
 }

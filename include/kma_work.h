@@ -88,6 +88,8 @@ typedef struct kma_run_stats {
 				   wavefront-round */
 	uint64_t deadline_hits; /* reference layout: mallocs that returned NULL
 				   because deadline_ms had passed */
+	uint64_t listed;        /* native: detached superblocks put into a partial index */
+	uint64_t reattached;    /* native: listed superblocks attached to a head again */
 } kma_run_stats;
 
 /* Run a per-work-item workload on the heap. */
@@ -206,6 +208,7 @@ typedef struct kma_stream_stats {
 	double tick_ns;        /* wall-clock tick length for kma_stream_times */
 	double phase_ns[3];    /* KMA_VAR_PHASES: malloc / free-target wait / free time per
 				  wavefront slice (plain: per wavefront launch) */
+	uint64_t listed, reattached; /* heap counters, per replay: partial-index puts and re-attachments */
 } kma_stream_stats;
 
 /* Run the stream on a native heap.
@@ -270,16 +273,34 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 /* Jobs arrive as a Poisson process and run for a log-normal number of
  * steps; each step appends one block of block_bytes to every live job, and
  * a job's blocks are all freed when it completes.  One device-resident
- * persistent kernel admits arrived jobs (oldest first) while the heap's use
- * is below watermark_pct and a job slot is free, grows every live job by one
- * block per step with one work-group collective malloc per 256 slots, and
- * frees completed jobs' blocks with the whole grid.  A job whose malloc
- * returns NULL waits a step; after a step with such stalls the most recently
- * admitted jobs are preempted (blocks freed, job requeued ahead of new
- * arrivals, restarting from step 0) until their blocks cover the shortfall,
- * and admission pauses until use falls 10 points below watermark_pct.
+ * persistent kernel admits arrived jobs (oldest first) while a job slot is
+ * free and the admission policy allows, grows every live job by one block
+ * per step with one work-group collective malloc per 256 slots, and frees
+ * completed jobs' blocks with the whole grid.
+ *
+ * Admission policies (spec.admit):
+ *   KMA_SERVE_ADMIT_PROJECTED (default)  every job states its length when it
+ *     arrives, so the heap use after each future step is known for the live
+ *     set: a timeline of projected use per step ahead (up to
+ *     KMA_SERVE_MAX_PROJECTED_LEN steps, kept in LDS by the work-group that
+ *     admits).  A job is admitted only if adding its growth (1, 2, ..., L
+ *     blocks over its L steps) keeps every future step at or below
+ *     watermark_pct of the heap's blocks; the first arrival that does not fit
+ *     waits, and so do all later ones (first come, first served).  Live jobs
+ *     then never find the heap full.
+ *   KMA_SERVE_ADMIT_WATERMARK  admit while the superblocks in use are below
+ *     watermark_pct of the heap (current use only); a step with stalls
+ *     pauses admission until use falls 10 points below the watermark.
+ * Under either policy a job whose malloc returns NULL waits a step; after a
+ * step with such stalls the most recently admitted jobs are preempted
+ * (blocks freed, job requeued ahead of new arrivals, restarting from step
+ * 0) until their blocks cover the shortfall (projected admission then
+ * rebuilds its timeline from the live set and admits nothing that step).
  * Job j's result is the sum of the 64-bit patterns its blocks carried in the
  * run that completed it, which depends on (j, its length) only. */
+#define KMA_SERVE_ADMIT_PROJECTED 0u
+#define KMA_SERVE_ADMIT_WATERMARK 1u
+#define KMA_SERVE_MAX_PROJECTED_LEN 8192u
 typedef struct kma_serve_spec {
 	double rate_per_ms;     /* mean arrivals per millisecond */
 	double len_mu;          /* log-normal length: exp(mu + sigma z) steps */
@@ -287,11 +308,15 @@ typedef struct kma_serve_spec {
 	double duration_ms;     /* wall-clock length of the run */
 	uint32_t max_len;       /* lengths are capped here (block-table width) */
 	uint32_t block_bytes;   /* appended per live job per step */
-	uint32_t watermark_pct; /* admit while heap use is below this percentage */
+	uint32_t watermark_pct; /* heap use limit of admission, percent (1..100; projected admission takes up
+				   to 400: over 100 it admits more than the heap holds and relies on preemption) */
 	uint32_t max_live;      /* job slots (multiple of 256) */
 	uint32_t max_admit;     /* admissions per step (0: 1024) */
 	uint32_t nwg;           /* work-groups (0: max(64, max_live / 256), co-resident) */
 	uint64_t seed;
+	uint32_t admit;         /* KMA_SERVE_ADMIT_*; projected needs max_len <= KMA_SERVE_MAX_PROJECTED_LEN */
+	uint32_t keep_live;     /* 1: end without freeing the blocks of jobs still live (the heap keeps them, for
+				   inspection with kma_snapshot / kma_heap_check; release it with kma_destroy) */
 } kma_serve_spec;
 
 typedef struct kma_serve_stats {
@@ -308,6 +333,20 @@ typedef struct kma_serve_stats {
 	double peak_used_frac;  /* most superblocks out of the pool at a step start */
 	uint32_t nwg;
 	uint32_t aborted;       /* a barrier watchdog fired */
+	uint32_t admit;         /* the admission policy run */
+	uint32_t pad;
+	uint64_t cap_blocks;    /* projected admission: the block limit (watermark_pct of the heap's blocks) */
+	uint64_t rebuilds;      /* projected admission: timeline rebuilds after steps with stalls */
+	uint64_t proj_checked;  /* projected admission: steps whose projected use was compared with the blocks
+				   held after their growth phase (steps without stalls or preemptions) */
+	uint64_t proj_mismatch; /* ... and found different (0 when the projection is exact) */
+	double goodput_per_s;   /* completed jobs per second of run_ms */
+	double preempt_frac;    /* preempted / admitted */
+	double mean_live;       /* live jobs per step, mean over the steps */
+	double peak_proj_frac;  /* projected admission: largest projected use of a step / the heap's blocks */
+	double phase_us[4];     /* per step, mean: admission work; admission + its grid barrier;
+				   growth + barrier; frees + barrier (wall clock of work-group 0) */
+	double busy_frac[2];    /* growth, frees: mean fraction of the phase the work-groups spend working */
 } kma_serve_stats;
 
 /* Run the serving loop on a native heap.  jobs == NULL: the whole trace;
@@ -419,7 +458,8 @@ typedef struct kma_heap_report {
 	uint64_t pages_used;
 	uint64_t pages_total;
 	uint64_t sb_unqueued;  /* reference layout: never queued, never used */
-	uint64_t counters[9];  /* native: the heap's event counters, KMA_STAT_* order;
+	uint64_t sb_listed;    /* native: detached, partly free, in a partial index */
+	uint64_t counters[11]; /* native: the heap's event counters, KMA_STAT_* order;
 				  reference: the stats trailer's words 0..7 */
 	char first[1024];      /* the first problems, one per line */
 } kma_heap_report;

@@ -96,6 +96,8 @@ class RunStats(ctypes.Structure):
         ("lat_max_ns", ctypes.c_double),
         ("phase_ns", ctypes.c_double * 3),
         ("deadline_hits", ctypes.c_uint64),
+        ("listed", ctypes.c_uint64),
+        ("reattached", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -139,6 +141,8 @@ class StreamStats(ctypes.Structure):
         ("remote_frees", ctypes.c_uint64),
         ("tick_ns", ctypes.c_double),
         ("phase_ns", ctypes.c_double * 3),
+        ("listed", ctypes.c_uint64),
+        ("reattached", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -177,7 +181,7 @@ class ServeSpec(ctypes.Structure):
         ("rate_per_ms", ctypes.c_double), ("len_mu", ctypes.c_double), ("len_sigma", ctypes.c_double),
         ("duration_ms", ctypes.c_double), ("max_len", ctypes.c_uint32), ("block_bytes", ctypes.c_uint32),
         ("watermark_pct", ctypes.c_uint32), ("max_live", ctypes.c_uint32), ("max_admit", ctypes.c_uint32),
-        ("nwg", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+        ("nwg", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("admit", ctypes.c_uint32), ("keep_live", ctypes.c_uint32),
     ]
 
 
@@ -194,11 +198,11 @@ class GraphStats(ctypes.Structure):
 class HeapReport(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in (
         "problems", "sb_total", "sb_free", "sb_in_spares", "sb_heads", "sb_detached", "orphans", "live_blocks",
-        "live_bytes", "page_runs", "pages_used", "pages_total", "sb_unqueued")] + [
-        ("counters", ctypes.c_uint64 * 9), ("first", ctypes.c_char * 1024)]
+        "live_bytes", "page_runs", "pages_used", "pages_total", "sb_unqueued", "sb_listed")] + [
+        ("counters", ctypes.c_uint64 * 11), ("first", ctypes.c_char * 1024)]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_[:13]}
+        d = {k: getattr(self, k) for k, _ in self._fields_[:14]}
         d["counters"] = list(self.counters)
         d["first"] = self.first.decode(errors="replace").splitlines()
         return d
@@ -210,16 +214,28 @@ class ServeStats(ctypes.Structure):
                                        "frees", "stalls", "drain_frees", "bad", "preempted")] + [
         (k, ctypes.c_double) for k in ("ops_per_s", "lat_p50_ns", "lat_p99_ns", "lat_max_ns", "wait_p50_us",
                                        "wait_p99_us", "peak_used_frac")] + [
-        ("nwg", ctypes.c_uint32), ("aborted", ctypes.c_uint32)]
+        ("nwg", ctypes.c_uint32), ("aborted", ctypes.c_uint32), ("admit", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+        ("cap_blocks", ctypes.c_uint64), ("rebuilds", ctypes.c_uint64), ("proj_checked", ctypes.c_uint64),
+        ("proj_mismatch", ctypes.c_uint64)] + [
+        (k, ctypes.c_double) for k in ("goodput_per_s", "preempt_frac", "mean_live", "peak_proj_frac")] + [
+        ("phase_us", ctypes.c_double * 4), ("busy_frac", ctypes.c_double * 2)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "phase_us", "busy_frac")}
+        d["phase_us"] = dict(zip(("admit_work", "admit", "grow", "free"), self.phase_us))
+        d["busy_frac"] = dict(zip(("grow", "free"), self.busy_frac))
+        return d
+
+
+ADMIT_PROJECTED = 0
+ADMIT_WATERMARK = 1
 
 
 def serve_spec(rate_per_ms=2000.0, len_mu=6.0, len_sigma=1.0, duration_ms=10000.0, max_len=8192,
-               block_bytes=16384, watermark_pct=90, max_live=32768, max_admit=0, nwg=0, seed=4):
+               block_bytes=16384, watermark_pct=90, max_live=32768, max_admit=0, nwg=0, seed=4,
+               admit=ADMIT_PROJECTED, keep_live=0):
     return ServeSpec(rate_per_ms, len_mu, len_sigma, duration_ms, max_len, block_bytes, watermark_pct, max_live,
-                     max_admit, nwg, seed)
+                     max_admit, nwg, seed, admit, keep_live)
 
 
 class KmaError(RuntimeError):

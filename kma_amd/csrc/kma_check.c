@@ -4,9 +4,10 @@
  * tb_kma.c:304-313) for both layouts.  Works on an image taken while no
  * kernel uses the heap.  Native layout (kma_snapshot of bytes
  * [0, off_data)): every superblock must be in exactly one place (a pool
- * shard, a context spare line, a context head, or in use with live
- * blocks), its count must equal its live bits, and the page bitmap must
- * match the run lengths and their window summaries.  Reference layout (the
+ * shard, a context spare line, a context head, the partial index of its
+ * home XCD and class, or in use with live blocks), its count must equal its
+ * live bits, and the page bitmap must match the run lengths and their
+ * window summaries.  Reference layout (the
  * whole image): the free queue is walked as the reference's validator does
  * (tb_clIndexedQueue.c:131-168) and every superblock's state word, block
  * size and bitmap must agree with where it is (queued, a class head, or
@@ -28,7 +29,9 @@
 #define ST_COUNT 0xffffffffull
 #define ST_CLASS_SHIFT 32
 #define ST_OWNED (1ull << 37)
-#define ST_GEN_SHIFT 41
+#define ST_HOME_SHIFT 38
+#define ST_LISTED (1ull << 41)
+#define ST_GEN_SHIFT 42
 #define POOL_STRIDE_WORDS 16u
 #define SPARE_WORDS 32u /* u32 per context spare line */
 
@@ -310,9 +313,12 @@ int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 		{h.off_stats, NAT_STATS_BYTES},
 		{h.off_psum, (uint64_t)NAT_PAGE_LEVELS * (ns_ + ns2_) * 8},
 		{h.off_spare, (uint64_t)h.nctx * 128},
+		{h.off_pidx, 8ull * h.nclasses * h.pidx_words * 8},
 	};
 	int bad = h.nsb > 0xfffffff0ull || h.npages > (1ull << 40) || h.bm_words > 4096 || h.npool > 512 ||
 		  h.nctx > 65536;
+	const uint64_t pw0 = (h.nsb + 63) >> 6, pw1 = (pw0 + 63) >> 6, pw2 = (pw1 + 63) >> 6;
+	bad = bad || h.pidx_words < pw0 + pw1 + pw2;
 	for (unsigned i = 0; i < sizeof(tab) / sizeof(tab[0]) && !bad; i++)
 		bad = tab[i].off > h.off_data || tab[i].len > h.off_data - tab[i].off;
 	if (bad) {
@@ -323,12 +329,47 @@ int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 	const uint64_t nsb = h.nsb, W = h.bm_words, S = h.npool / 8;
 	uint8_t *where = (uint8_t *)calloc(nsb + 1, 1); /* free places holding each superblock */
 	uint8_t *headed = (uint8_t *)calloc(nsb + 1, 1);
-	if (!where || !headed) {
+	uint8_t *listed = (uint8_t *)calloc(nsb + 1, 1);    /* partial-index bits naming it */
+	uint16_t *lplace = (uint16_t *)calloc(nsb + 1, 2); /* (XCD << 8) | class of that index */
+	if (!where || !headed || !listed || !lplace) {
 		free(where);
 		free(headed);
+		free(listed);
+		free(lplace);
 		set_err("host allocation failed");
 		return -1;
 	}
+	/* partial indexes: level-0 bits name listed superblocks; a non-zero
+	 * word must have its bit set one level up (set bits are hints) */
+	for (uint32_t x = 0; x < 8; x++)
+		for (uint32_t c = 0; c < h.nclasses; c++) {
+			const uint64_t P = h.off_pidx + ((uint64_t)x * h.nclasses + c) * h.pidx_words * 8;
+			for (uint64_t i = 0; i < pw0; i++) {
+				uint64_t w = rd64(img, P + i * 8);
+				if (!w)
+					continue;
+				if (!((rd64(img, P + (pw0 + (i >> 6)) * 8) >> (i & 63)) & 1ull))
+					problem(&p, "partial index of XCD %u class %u: word %llu is not zero but its summary bit is "
+						    "clear",
+						x, c, (unsigned long long)i);
+				for (; w; w &= w - 1) {
+					uint64_t sb = i * 64 + (uint64_t)__builtin_ctzll(w);
+					if (sb >= nsb) {
+						problem(&p, "partial index of XCD %u class %u names superblock %llu, past the heap", x,
+							c, (unsigned long long)sb);
+						continue;
+					}
+					if (listed[sb] < 255)
+						listed[sb]++;
+					lplace[sb] = (uint16_t)((x << 8) | c);
+				}
+			}
+			for (uint64_t i = 0; i < pw1; i++)
+				if (rd64(img, P + (pw0 + i) * 8) && !((rd64(img, P + (pw0 + pw1 + (i >> 6)) * 8) >> (i & 63)) & 1ull))
+					problem(&p, "partial index of XCD %u class %u: summary word %llu is not zero but its top bit "
+						    "is clear",
+						x, c, (unsigned long long)i);
+		}
 	/* pool shards */
 	for (uint32_t s = 0; s < h.npool; s++) {
 		uint64_t hp = h.off_pool + (uint64_t)s * POOL_STRIDE_WORDS * 8;
@@ -402,8 +443,11 @@ int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 				if (((b >> j) & 1u) && w * 32 + j >= cap)
 					beyond++;
 		}
+		if (listed[sb] > 1 || (listed[sb] != 0) != ((st & ST_LISTED) != 0))
+			problem(&p, "superblock %llu: listed bit %d, in %u partial indexes", (unsigned long long)sb,
+				(st & ST_LISTED) != 0, listed[sb]);
 		if (c1 == 0) {
-			if (cnt || live || (st & ST_OWNED))
+			if (cnt || live || (st & (ST_OWNED | ST_LISTED)))
 				problem(&p, "free superblock %llu: state %#llx, %llu bits set", (unsigned long long)sb,
 					(unsigned long long)st, (unsigned long long)live);
 			if (where[sb] != 1)
@@ -426,7 +470,15 @@ int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 		if (((st & ST_OWNED) != 0) != (headed[sb] == 1))
 			problem(&p, "superblock %llu: owned bit %d, named by %u heads", (unsigned long long)sb,
 				(st & ST_OWNED) != 0, headed[sb]);
-		if (!(st & ST_OWNED)) {
+		if (st & ST_LISTED) {
+			const uint32_t home = (uint32_t)(st >> ST_HOME_SHIFT) & 7u;
+			if (st & ST_OWNED)
+				problem(&p, "superblock %llu is listed and owned", (unsigned long long)sb);
+			else if (listed[sb] == 1 && lplace[sb] != (uint16_t)((home << 8) | (c1 - 1u)))
+				problem(&p, "superblock %llu of XCD %u class %u is in the partial index of XCD %u class %u",
+					(unsigned long long)sb, home, c1 - 1u, lplace[sb] >> 8, lplace[sb] & 0xffu);
+			out->sb_listed++;
+		} else if (!(st & ST_OWNED)) {
 			if (cnt == 0) {
 				out->orphans++;
 				problem(&p, "superblock %llu of class %u has no live block, no owner and is in no free "
@@ -485,6 +537,8 @@ int kma_image_check(const void *image, uint64_t bytes, kma_heap_report *out)
 		out->counters[k] = rd64(img, h.off_stats + 8ull * k);
 	free(where);
 	free(headed);
+	free(listed);
+	free(lplace);
 	return 0;
 }
 

@@ -267,7 +267,7 @@ int kma_trim(kma_heap *h)
 {
 	if (!h || h->cfg.layout != KMA_LAYOUT_NATIVE)
 		return 0;
-	LCHK(kmad_trim(h->dev, h->nh.nctx, h->nh.nclasses), -1);
+	LCHK(kmad_trim(h->dev, h->nh.nctx, h->nh.nclasses, (uint32_t)((h->nh.nsb + 63) / 64)), -1);
 	return 0;
 }
 
@@ -461,6 +461,8 @@ int kma_run_workload(kma_heap *h, const kma_workload *wl, kma_run_stats *out)
 	}
 	out->remote_frees = st1[7] - st0[7];
 	out->migrated = st1[8] - st0[8];
+	out->listed = st1[9] - st0[9];
+	out->reattached = st1[10] - st0[10];
 	double tick = kma_i_wall_clock_ns();
 	if (hs[7])
 		for (int k = 0; k < 3; k++)

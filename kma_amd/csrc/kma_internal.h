@@ -11,7 +11,7 @@
 #include "kma.h"
 #include "kma_layout.h"
 
-#define NAT_NSTATS 9u
+#define NAT_NSTATS 11u
 #define NAT_STAT_DOUBLE_FREE 0
 #define NAT_STAT_BAD_FREE 1
 

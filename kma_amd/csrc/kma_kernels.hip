@@ -136,8 +136,30 @@ __global__ void k_trim(kma_dheap *H, uint32_t nctx, uint32_t nclasses)
 		return;
 	uint32_t sb = (uint32_t)h - 2u;
 	uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
-	if (st_count(s) == 0 && st_cls1(s) != 0)
-		try_release(H, sb, s, st_home(s), ctx);
+	after_detach(H, sb, s, st_home(s), ctx);
+}
+
+/* Partial index (no allocation may be in flight): every listed superblock
+ * that holds no live block is taken out of the index and released (into a
+ * dirty shard of its home or a spare line, drained by k_clean_dirty).  One
+ * thread per level-0 word; then k_pidx_rebuild, one thread per (XCD,
+ * class), rebuilds the summary levels. */
+__global__ void k_trim_pidx(kma_dheap *H, uint32_t nwords)
+{
+	using namespace kma_nat;
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nwords)
+		return;
+	const uint32_t w0 = pidx_w0(H), i0 = i % w0, c = (i / w0) % H->nclasses, x = i / (w0 * H->nclasses);
+	trim_pidx_word(H, x, c, i0);
+}
+
+__global__ void k_pidx_rebuild(kma_dheap *H)
+{
+	using namespace kma_nat;
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < KMA_NXCD * H->nclasses)
+		pidx_rebuild(H, i / H->nclasses, i % H->nclasses);
 }
 
 /* After a kernel boundary: drain dirty shards and spare lines into clean
@@ -149,10 +171,17 @@ __global__ void k_clean_dirty(kma_dheap *H)
 	kma_nat::clean_all(H);
 }
 
-extern "C" int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses)
+extern "C" int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses, uint32_t pidx_w0)
 {
 	uint32_t n = nctx * nclasses;
 	hipLaunchKernelGGL(k_trim, dim3((n + 255) / 256), dim3(256), 0, 0, (kma_dheap *)heap, nctx, nclasses);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	uint32_t nw = KMA_NXCD * nclasses * pidx_w0;
+	hipLaunchKernelGGL(k_trim_pidx, dim3((nw + 255) / 256), dim3(256), 0, 0, (kma_dheap *)heap, nw);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	hipLaunchKernelGGL(k_pidx_rebuild, dim3(1), dim3(256), 0, 0, (kma_dheap *)heap);
 	KMA_CHECK(hipGetLastError());
 	KMA_CHECK(hipDeviceSynchronize());
 	hipLaunchKernelGGL(k_clean_dirty, dim3(1), dim3(64), 0, 0, (kma_dheap *)heap);

@@ -37,7 +37,7 @@ int kmad_batched(void *heap, uint32_t round0, uint32_t nrounds, int mode,
 		 uint32_t fixed, uint64_t seed, uint64_t nwi, uint32_t variant,
 		 unsigned long long *d_stats, uint32_t *d_lat, void *stream);
 
-int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses);
+int kmad_trim(void *heap, uint32_t nctx, uint32_t nclasses, uint32_t pidx_w0);
 
 int kmad_wave_free(void *heap, const uint64_t *offs, uint32_t n);
 int kmad_check_wave_claim(void *heap, uint32_t ncases, const uint32_t *cls, const uint32_t *base,
@@ -95,12 +95,16 @@ typedef struct kma_serve_args {
 	uint32_t block_bytes;
 	uint32_t watermark_pct;
 	uint32_t max_admit;       /* admissions per step */
-	uint32_t pad;
+	uint32_t admit;           /* KMA_SERVE_ADMIT_* */
 	uint64_t duration_ticks;
+	uint64_t cap_blocks;      /* projected admission: block limit (< 2^31) */
+	uint32_t keep_live;       /* skip the drain */
+	uint32_t pad2;
 	int64_t *slot_job;        /* -1: free */
 	uint32_t *slot_steps;
 	uint64_t *table;          /* [max_live][max_len] block addresses */
-	uint32_t *free_slots;     /* stack, max_live entries */
+	uint32_t *free_slots;     /* one stack per XCD, max_live entries each */
+	uint32_t *wg_xcd;         /* per work-group: the XCD it runs on */
 	uint64_t *ctl;
 	uint64_t *done;           /* [max_live][4]: slot, job, first, count */
 	uint64_t *result;         /* per job: sum of its blocks' patterns */

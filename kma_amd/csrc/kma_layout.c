@@ -141,6 +141,15 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 		uint64_t q = nh->nsb / (4ull * nh->nctx);
 		nh->spare_slots = (uint32_t)(q < 16 ? q : 16);
 	}
+	/* partial index (detached, partly free superblocks waiting to be
+	 * attached again): per XCD and class, a bit per superblock and two
+	 * summary levels, each block on its own 128-B lines */
+	{
+		uint64_t w0 = (nh->nsb + 63) / 64, w1 = (w0 + 63) / 64, w2 = (w1 + 63) / 64;
+		nh->pidx_words = (uint32_t)align_up(w0 + w1 + w2, 16);
+	}
+	nh->off_pidx = off;
+	off = align_up(off + 8ull * nh->nclasses * nh->pidx_words * 8ull, 256);
 	/* metadata bytes; data placed after allocation so it is absolutely
 	 * aligned to the superblock and page sizes */
 	nh->off_data = off;

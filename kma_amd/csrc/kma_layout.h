@@ -45,6 +45,9 @@ typedef struct nat_hdr {
 	uint64_t off_spare;
 	uint32_t spare_slots;
 	uint32_t pad1;
+	uint64_t off_pidx;
+	uint32_t pidx_words;
+	uint32_t pad2;
 } nat_hdr;
 
 #define NAT_MAGIC 0x4e414d4b31414d4bull

@@ -16,7 +16,6 @@
 #include "kma_work.h"
 
 #define SV_C_NEXT 32
-#define SV_C_NFREE 48
 #define SV_C_ABORT 112
 
 static uint64_t smix(uint64_t seed, uint64_t a, uint64_t b)
@@ -169,11 +168,32 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	kma_serve_spec sp = *spec;
 	if (sp.max_admit == 0)
 		sp.max_admit = 1024;
-	if (sp.max_live == 0 || (sp.max_live & 255u) || sp.max_len == 0 || sp.block_bytes == 0 ||
-	    sp.watermark_pct == 0 || sp.watermark_pct > 100 || sp.duration_ms <= 0 || (!jobs && sp.rate_per_ms <= 0)) {
+	if (sp.max_live == 0 || (sp.max_live & 255u) || sp.max_live > 1024u * 256u || sp.max_len == 0 || sp.block_bytes == 0 ||
+	    sp.watermark_pct == 0 || sp.watermark_pct > (sp.admit == KMA_SERVE_ADMIT_PROJECTED ? 400u : 100u) ||
+	    sp.duration_ms <= 0 || (!jobs && sp.rate_per_ms <= 0) || sp.admit > KMA_SERVE_ADMIT_WATERMARK) {
 		set_err("serving loop: bad spec");
 		return -1;
 	}
+	if (sp.admit == KMA_SERVE_ADMIT_PROJECTED && sp.max_len > KMA_SERVE_MAX_PROJECTED_LEN) {
+		set_err("serving loop: projected admission looks at most %u steps ahead (max_len %u)",
+			KMA_SERVE_MAX_PROJECTED_LEN, sp.max_len);
+		return -1;
+	}
+	/* blocks of block_bytes the heap holds: superblocks x the slots of the
+	 * smallest class that fits a block */
+	uint64_t heap_blocks = 0;
+	for (uint32_t c = 0; c < h->nh.nclasses; c++)
+		if (h->nh.class_size[c] >= sp.block_bytes) {
+			heap_blocks = h->nh.nsb * (uint64_t)h->nh.class_cap[c];
+			break;
+		}
+	if (heap_blocks == 0) {
+		set_err("serving loop: no size class holds %u-byte blocks", sp.block_bytes);
+		return -1;
+	}
+	uint64_t cap_blocks = heap_blocks * sp.watermark_pct / 100u;
+	if (cap_blocks > (1ull << 31))
+		cap_blocks = 1ull << 31; /* timeline entries are u32 */
 	memset(out, 0, sizeof(*out));
 	double tick_ns = kma_i_wall_clock_ns(), ticks_per_ms = 1e6 / tick_ns;
 
@@ -212,20 +232,24 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	a.block_bytes = sp.block_bytes;
 	a.watermark_pct = sp.watermark_pct;
 	a.max_admit = sp.max_admit;
+	a.admit = sp.admit;
+	a.cap_blocks = cap_blocks;
+	a.keep_live = sp.keep_live != 0;
 	a.duration_ticks = (uint64_t)(sp.duration_ms * ticks_per_ms);
-	void *bufs[16] = {0};
+	void *bufs[17] = {0};
 	uint64_t nn = n ? n : 1, rq = (uint64_t)sp.max_live * 4;
-	uint64_t sizes[16] = {nn * 8, nn * 4, (uint64_t)sp.max_live * 8, (uint64_t)sp.max_live * 4,
-			      (uint64_t)sp.max_live * sp.max_len * 8, (uint64_t)sp.max_live * 4, KMA_SV_CTL_WORDS * 8,
+	/* [5]: one free-slot stack per XCD, filled on the device (the XCD of each
+	 * work-group is known only there); [16]: each work-group's XCD */
+	uint64_t sizes[17] = {nn * 8, nn * 4, (uint64_t)sp.max_live * 8, (uint64_t)sp.max_live * 4,
+			      (uint64_t)sp.max_live * sp.max_len * 8, (uint64_t)sp.max_live * 4 * 8, KMA_SV_CTL_WORDS * 8,
 			      (uint64_t)sp.max_live * 32, nn * 8, nn * 8, nn * 8, KMA_SV_NHIST * 8,
-			      (uint64_t)sp.max_live * 4, (uint64_t)sp.max_live * 16, rq * 8, nn * 8};
+			      (uint64_t)sp.max_live * 4, (uint64_t)sp.max_live * 16, rq * 8, nn * 8, (uint64_t)nwg * 4};
 	int rc = -1;
 	hipStream_t st = NULL;
 	hipEvent_t e0 = NULL, e1 = NULL;
 	uint64_t *hist = NULL, *adm = NULL, *waits = NULL;
-	uint32_t *fs = NULL;
-	for (int i = 0; i < (jobs ? 16 : 15); i++)
-		if (hipMalloc(&bufs[i], sizes[i]) != hipSuccess) {
+	for (int i = 0; i < 17; i++)
+		if ((i != 15 || jobs) && hipMalloc(&bufs[i], sizes[i]) != hipSuccess) {
 			set_err("serving loop: device allocation of %llu bytes failed", (unsigned long long)sizes[i]);
 			goto done;
 		}
@@ -247,20 +271,14 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	a.requeue = (uint64_t *)bufs[14];
 	a.rq_cap = rq;
 	a.job_id = jobs ? (const uint64_t *)bufs[15] : NULL;
-	fs = (uint32_t *)malloc((size_t)sp.max_live * 4);
-	if (!fs)
-		goto done;
-	for (uint32_t i = 0; i < sp.max_live; i++)
-		fs[i] = sp.max_live - 1u - i; /* slot 0 on top */
+	a.wg_xcd = (uint32_t *)bufs[16];
 	uint64_t ctl0[KMA_SV_CTL_WORDS];
 	memset(ctl0, 0, sizeof(ctl0));
-	ctl0[SV_C_NFREE] = sp.max_live;
 	if ((jobs && hipMemcpy(bufs[15], jobs, n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
 	    hipMemset(bufs[12], 0, sizes[12]) != hipSuccess ||
 	    (n && hipMemcpy(bufs[0], arr, n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
 	    (n && hipMemcpy(bufs[1], len, n * 4, hipMemcpyHostToDevice) != hipSuccess) ||
 	    hipMemset(bufs[2], 0xff, sizes[2]) != hipSuccess || hipMemset(bufs[3], 0, sizes[3]) != hipSuccess ||
-	    hipMemcpy(bufs[5], fs, sizes[5], hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemcpy(bufs[6], ctl0, sizeof(ctl0), hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemset(bufs[8], 0, sizes[8]) != hipSuccess || hipMemset(bufs[9], 0, sizes[9]) != hipSuccess ||
 	    hipMemset(bufs[10], 0, sizes[10]) != hipSuccess || hipMemset(bufs[11], 0, sizes[11]) != hipSuccess ||
@@ -307,7 +325,28 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	out->drained = S[9];
 	out->drain_frees = S[10];
 	out->preempted = S[11];
+	out->rebuilds = S[12];
+	out->proj_checked = S[21];
+	out->proj_mismatch = S[22];
+	out->admit = sp.admit;
+	out->cap_blocks = cap_blocks;
+	out->peak_proj_frac = (double)S[13] / (double)heap_blocks;
+	out->mean_live = out->steps ? (double)S[14] / (double)out->steps : 0;
+	{
+		/* phase profile: S[15..18] work-group 0's wall clock per phase over
+		 * steps + 1 phase-A passes (the last one decides to stop) and steps
+		 * B and C phases; S[19], S[20] all groups' working time in B, C */
+		double na = (double)(out->steps + 1u), nb = out->steps ? (double)out->steps : 1.0;
+		out->phase_us[0] = S[15] * tick_ns * 1e-3 / na;
+		out->phase_us[1] = S[16] * tick_ns * 1e-3 / na;
+		out->phase_us[2] = S[17] * tick_ns * 1e-3 / nb;
+		out->phase_us[3] = S[18] * tick_ns * 1e-3 / nb;
+		out->busy_frac[0] = S[17] ? (double)S[19] / ((double)S[17] * nwg) : 0;
+		out->busy_frac[1] = S[18] ? (double)S[20] / ((double)S[18] * nwg) : 0;
+	}
 	out->ops_per_s = out->run_ms > 0 ? (double)(out->mallocs + out->frees) / (out->run_ms * 1e-3) : 0;
+	out->goodput_per_s = out->run_ms > 0 ? (double)out->done / (out->run_ms * 1e-3) : 0;
+	out->preempt_frac = out->admitted ? (double)out->preempted / (double)out->admitted : 0;
 	out->lat_p50_ns = hist_pct(hist, KMA_SV_NHIST, 0.50, tick_ns);
 	out->lat_p99_ns = hist_pct(hist, KMA_SV_NHIST, 0.99, tick_ns);
 	out->lat_max_ns = hist_pct(hist, KMA_SV_NHIST, 1.0, tick_ns);
@@ -350,7 +389,6 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 done:
 	free(arr);
 	free(len);
-	free(fs);
 	free(hist);
 	free(adm);
 	free(waits);
@@ -360,7 +398,7 @@ done:
 		hipEventDestroy(e1);
 	if (st)
 		hipStreamDestroy(st);
-	for (int i = 0; i < 16; i++)
+	for (int i = 0; i < 17; i++)
 		if (bufs[i])
 			hipFree(bufs[i]);
 	return rc;

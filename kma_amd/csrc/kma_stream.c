@@ -428,6 +428,8 @@ int kma_stream_run(kma_heap *h, kma_stream *s, uint32_t form, uint32_t nwg, uint
 	for (int k = 0; k < 4; k++)
 		out->waits[k] = (st1[3 + k] - st0[3 + k]) / reps;
 	out->remote_frees = (st1[7] - st0[7]) / reps;
+	out->listed = (st1[9] - st0[9]) / reps;
+	out->reattached = (st1[10] - st0[10]) / reps;
 	out->tick_ns = kma_i_wall_clock_ns();
 	rc = 0;
 done:

@@ -42,8 +42,13 @@ static void trim(kma_dheap *H)
 				continue;
 			uint32_t sb = (uint32_t)h - 2u;
 			uint64_t s = f_and(state(H, sb), ~ST_OWNED) & ~ST_OWNED;
-			if (st_count(s) == 0 && st_cls1(s) != 0)
-				try_release(H, sb, s, st_home(s), ctx);
+			after_detach(H, sb, s, st_home(s), ctx);
+		}
+	for (uint32_t x = 0; x < KMA_NXCD; x++)
+		for (uint32_t c = 0; c < H->nclasses; c++) {
+			for (uint32_t i = 0; i < pidx_w0(H); i++)
+				trim_pidx_word(H, x, c, i);
+			pidx_rebuild(H, x, c);
 		}
 	clean_all(H);
 	for (uint32_t ctx = 0; ctx < H->nctx; ctx++)
@@ -57,7 +62,7 @@ static int canonical_problems(kma_dheap *H)
 	using namespace kma_nat;
 	int probs = 0;
 	for (uint64_t sb = 0; sb < H->nsb; sb++) {
-		if (*state(H, sb) & ((1ull << 41) - 1)) {
+		if (*state(H, sb) & ((1ull << 42) - 1)) { /* all but the generation */
 			if (probs < 5)
 				fprintf(stderr, "sb %llu state %#llx\n", (unsigned long long)sb,
 					(unsigned long long)*state(H, sb));
@@ -88,6 +93,15 @@ static int canonical_problems(kma_dheap *H)
 	}
 	if (n != H->nsb)
 		probs++;
+	/* partial indexes empty */
+	for (uint32_t x = 0; x < KMA_NXCD; x++)
+		for (uint32_t c = 0; c < H->nclasses; c++)
+			for (uint32_t i = 0; i < H->pidx_words; i++)
+				if (pidx(H, x, c)[i]) {
+					fprintf(stderr, "xcd %u class %u partial-index word %u = %#llx\n", x, c, i,
+						(unsigned long long)pidx(H, x, c)[i]);
+					probs++;
+				}
 	/* every non-empty shard has its hint bit set */
 	uint32_t S = H->npool / KMA_NXCD;
 	for (uint32_t s = 0; s < H->npool; s++)

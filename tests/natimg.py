@@ -12,6 +12,7 @@ HDR_DTYPE = np.dtype([
     ("class_size", "<u4", 32), ("class_cap", "<u4", 32), ("class_magic", "<u8", 32),
     ("class_for_log2", "u1", 64),
     ("off_psum", "<u8"), ("off_spare", "<u8"), ("spare_slots", "<u4"), ("pad1", "<u4"),
+    ("off_pidx", "<u8"), ("pidx_words", "<u4"), ("pad2", "<u4"),
 ])
 
 MAGIC = 0x4E414D4B31414D4B
@@ -65,6 +66,7 @@ class NatHeap:
             "pool": arr(h["off_pool"], self.npool * POOL_STRIDE, "<u8").reshape(self.npool, POOL_STRIDE),
             "pbitmap": arr(h["off_pbitmap"], (self.npages + 63) // 64, "<u8"),
             "plen": arr(h["off_plen"], self.npages, "u1"),
+            "pidx": arr(h["off_pidx"], 8 * self.nclasses * int(h["pidx_words"]), "<u8"),
         }
 
     def pool_members(self, m):
@@ -87,8 +89,8 @@ class NatHeap:
     def canonical_problems(self):
         """A heap with no live block, after trim: every bitmap and page-bitmap
         word zero, every state word zero apart from its generation, every
-        context head and spare line empty, every superblock in exactly one
-        pool shard."""
+        context head, spare line and partial index empty,
+        every superblock in exactly one pool shard."""
         m = self.meta()
         probs = []
         if m["spare"].any():
@@ -97,11 +99,13 @@ class NatHeap:
             probs.append("%d bitmap words nonzero" % int(np.count_nonzero(m["bitmap"])))
         if m["pbitmap"].any():
             probs.append("page bitmap nonzero")
-        gen_mask = np.uint64((1 << 41) - 1)
+        gen_mask = np.uint64((1 << 42) - 1)  # all but the generation
         if (m["state"] & gen_mask).any():
             probs.append("%d state words not free" % int(np.count_nonzero(m["state"] & gen_mask)))
         if m["heads"].any():
             probs.append("context heads not empty")
+        if m["pidx"].any():
+            probs.append("%d partial-index words not zero" % int(np.count_nonzero(m["pidx"])))
         seen, p = self.pool_members(m)
         probs += p
         if sorted(seen) != list(range(self.nsb)):

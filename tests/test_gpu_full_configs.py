@@ -166,6 +166,11 @@ def test_serving_loop_full_size():
         assert st["mallocs"] == st["frees"] + st["drain_frees"], st
         assert st["admitted"] == st["done"] + st["drained"] + st["preempted"], st
         assert st["jobs"] == len(arr) and 9500 < st["run_ms"] < 10500, st
+        # projected admission: (almost) no preemption, and the projection of
+        # every step without stalls equals the blocks held after it
+        assert st["admit"] == kma_amd.ADMIT_PROJECTED and st["proj_mismatch"] == 0, st
+        assert st["preempted"] <= 0.1 * st["admitted"], st
+        assert st["proj_checked"] >= 0.9 * st["steps"], st
         done = np.nonzero(res)[0].astype(np.uint64)
         assert len(done) == st["done"] and st["done"] > 10000, st
         rng = np.random.default_rng(4)
