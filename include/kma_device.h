@@ -114,7 +114,7 @@ typedef struct kma_dheap {
 #define KMA_STAT_REMOTE_FREE 7  /* frees issued from a non-home XCD */
 #define KMA_STAT_MIGRATED 8     /* superblocks moved from dirty to clean shards in a kernel */
 #define KMA_STAT_LISTED 9       /* detached superblocks put into a partial index */
-#define KMA_STAT_REATTACHED 10  /* listed superblocks attached to a head again */
+#define KMA_STAT_REATTACHED 10  /* listed superblocks reused: attached to a head again, or grabbed */
 #define KMA_NSTATS 11
 /* The stats block is 256 B of counters and one 128-B line per XCD x holding
  * x's shard mask (bit k set when shard x*S + k may hold a superblock, see
@@ -713,15 +713,26 @@ __device__ __forceinline__ uint32_t pidx_level(const kma_dheap *H, uint32_t lv)
 {
 	return lv == 0u ? 0u : lv == 1u ? pidx_w0(H) : pidx_w0(H) + pidx_w1(H);
 }
-/* Free slots a detached superblock needs before it is listed. */
+/* Free slots a detached superblock needs before it is listed: an eighth of
+ * it, or a sixteenth while its XCD is under pressure (the free capacity of
+ * a nearly full heap is a few blocks per superblock; listing superblocks
+ * with fewer would make every take of the index yield a block or two). */
 __device__ __forceinline__ uint32_t part_room(uint32_t cap) { return cap >= 8u ? cap / 8u : 1u; }
+__device__ __forceinline__ uint32_t part_room_pressed(uint32_t cap) { return cap >= 16u ? cap / 16u : 1u; }
 
-/* Set bit i of level lv; a word that was zero sets its own bit one level up. */
+/* Set bit i of level lv; a word that was zero sets its own bit one level up
+ * (a summary bit that reads set already is not written again: the few
+ * summary words are the index's hottest). */
 __device__ inline void pidx_set(const kma_dheap *H, uint64_t *P, uint32_t lv, uint32_t i)
 {
-	for (; lv < 3u; lv++, i >>= 6)
-		if (f_or(P + pidx_level(H, lv) + (i >> 6), (uint64_t)1 << (i & 63u)) != 0)
+	for (; lv < 3u; lv++, i >>= 6) {
+		uint64_t *w = P + pidx_level(H, lv) + (i >> 6);
+		const uint64_t b = (uint64_t)1 << (i & 63u);
+		if (lv > 0u && (ld(w) & b))
 			return;
+		if (f_or(w, b) != 0)
+			return;
+	}
 }
 
 /* Word i of level lv (0 or 1) was seen zero (after the caller's RMW, or by a
@@ -808,8 +819,17 @@ __device__ inline bool pidx_remove(const kma_dheap *H, uint32_t x, uint32_t c, u
 	return true;
 }
 
+/* XCD x's last pool search found no empty superblock (its refills are
+ * taking listed superblocks). */
+__device__ __forceinline__ bool under_pressure(const kma_dheap *H, uint32_t x)
+{
+	return ld(pressure(H, x)) != 0;
+}
+
 /* Called by whoever produced state `s` == {count 0, not owned, not listed,
- * class set}, running on XCD x. */
+ * class set}, running on XCD x.  A local release keeps the superblock in
+ * the releasing context's spare line unless the XCD is under pressure:
+ * then every context of the XCD must find it, in the XCD's dirty shard. */
 __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t x, uint32_t ctx)
 {
 	uint32_t g = st_gen(s);
@@ -820,7 +840,7 @@ __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, 
 			/* a local release leaves this XCD's L2 as it is; a remote
 			 * freer has written its own L2 back (kma_free) */
 			uint32_t home = st_home(s);
-			if (home != x || !spare_put(H, ctx, sb))
+			if (home != x || under_pressure(H, x) || !spare_put(H, ctx, sb))
 				pool_push(H, dirty_shard(H, home, ctx), sb);
 			return;
 		}
@@ -830,21 +850,27 @@ __device__ inline void try_release(const kma_dheap *H, uint32_t sb, uint64_t s, 
 	}
 }
 
-/* XCD x's last pool search found no empty superblock (its refills are
- * taking listed superblocks). */
-__device__ __forceinline__ bool under_pressure(const kma_dheap *H, uint32_t x)
+/* Set listed superblock sb's index bit (s: its state, LISTED set).  A free
+ * that empties it before the bit is set finds no bit and leaves it to this
+ * call: the state is read again after the put, and if this incarnation
+ * (same generation, still listed, not owned) has emptied, its bit is taken
+ * back here and it is released (a refill, or that free, may have taken the
+ * bit first: then nothing is left to do). */
+__device__ inline void put_listed(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t x, uint32_t ctx)
 {
-	return ld(pressure(H, x)) != 0;
+	const uint32_t c = st_cls1(s) - 1u;
+	pidx_put(H, st_home(s), c, sb);
+	const uint64_t now = ld(state(H, sb));
+	if (st_count(now) == 0 && (now & ~ST_COUNT) == (s & ~ST_COUNT) && pidx_remove(H, st_home(s), c, sb)) {
+		const uint64_t r = f_and(state(H, sb), ~ST_LISTED) & ~ST_LISTED;
+		if (st_count(r) == 0 && !(r & ST_OWNED))
+			try_release(H, sb, r, x, ctx);
+	}
 }
 
 /* List detached superblock sb (state s as last seen: not owned, class set,
  * count in 1..lim): set LISTED with a CAS while that still holds, then set
- * its bit in the index of (home, class).  A free that empties it before the
- * bit is set finds no bit and leaves it to this call: the state is read
- * again after the put, and if this incarnation (same generation, still
- * listed, not owned) has emptied, its bit is taken back here and it is
- * released (a refill, or that free, may have taken the bit first: then
- * nothing is left to do). */
+ * its bit in the index of (home, class) (put_listed). */
 __device__ inline void try_list(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t lim, uint32_t x, uint32_t ctx)
 {
 	const uint32_t c1 = st_cls1(s);
@@ -856,15 +882,28 @@ __device__ inline void try_list(const kma_dheap *H, uint32_t sb, uint64_t s, uin
 			break;
 		s = seen;
 	}
-	s |= ST_LISTED;
-	pidx_put(H, st_home(s), c1 - 1u, sb);
 	bump_stat(H, KMA_STAT_LISTED);
-	const uint64_t now = ld(state(H, sb));
-	if (st_count(now) == 0 && (now & ~ST_COUNT) == (s & ~ST_COUNT) &&
-	    pidx_remove(H, st_home(s), c1 - 1u, sb)) {
-		const uint64_t r = f_and(state(H, sb), ~ST_LISTED) & ~ST_LISTED;
-		if (st_count(r) == 0 && !(r & ST_OWNED))
-			try_release(H, sb, r, x, ctx);
+	put_listed(H, sb, s | ST_LISTED, x, ctx);
+}
+
+/* A listed superblock just taken out of the index by a partial grab
+ * (LISTED still set) goes back in if it still has room to offer (an eighth
+ * of it free, or a sixteenth while its XCD is under pressure: press, as the
+ * caller read it); otherwise its LISTED bit is cleared, and its frees list
+ * it again when they cross to that much room. */
+__device__ inline void relist(const kma_dheap *H, uint32_t sb, uint32_t x, uint32_t ctx, bool press)
+{
+	uint64_t s = ld(state(H, sb));
+	for (;;) {
+		const uint32_t cap = H->class_cap[st_cls1(s) - 1u], cnt = st_count(s);
+		if (cnt <= cap - part_room(cap) || (press && cnt <= cap - part_room_pressed(cap))) {
+			put_listed(H, sb, s, x, ctx);
+			return;
+		}
+		const uint64_t seen = cas(state(H, sb), s, s & ~ST_LISTED);
+		if (seen == s)
+			return;
+		s = seen;
 	}
 }
 
@@ -885,9 +924,8 @@ __device__ inline void unlist_release(const kma_dheap *H, uint32_t sb, uint64_t 
 
 /* After a count decrement of m on superblock sb left state ns: release it
  * if it emptied with no owner; list it, while detached, when it crosses to
- * an eighth free, or at its first free while its home XCD's pool is
- * exhausted (the free capacity of a nearly full heap is a few blocks per
- * superblock, all of which must stay reachable). */
+ * an eighth free, or at any free that leaves a sixteenth free while its
+ * home XCD's pool is exhausted. */
 __device__ __forceinline__ void after_sub(const kma_dheap *H, uint32_t sb, uint64_t ns, uint32_t m, uint32_t x,
 					 uint32_t ctx)
 {
@@ -906,8 +944,8 @@ __device__ __forceinline__ void after_sub(const kma_dheap *H, uint32_t sb, uint6
 	const uint32_t cap = H->class_cap[st_cls1(ns) - 1u], lim = cap - part_room(cap);
 	if (cnt <= lim && cnt + m > lim)
 		try_list(H, sb, ns, lim, x, ctx);
-	else if (cnt > lim && cnt + m >= cap && under_pressure(H, st_home(ns)))
-		try_list(H, sb, ns, cap - 1u, x, ctx);
+	else if (cnt > lim && cnt <= cap - part_room_pressed(cap) && under_pressure(H, st_home(ns)))
+		try_list(H, sb, ns, cap - part_room_pressed(cap), x, ctx);
 }
 
 /* After the owned bit was cleared (state s): release or list. */
@@ -920,8 +958,8 @@ __device__ __forceinline__ void after_detach(const kma_dheap *H, uint32_t sb, ui
 		try_release(H, sb, s, x, ctx);
 	else if (cnt <= cap - part_room(cap))
 		try_list(H, sb, s, cap - part_room(cap), x, ctx);
-	else if (cnt < cap && under_pressure(H, st_home(s)))
-		try_list(H, sb, s, cap - 1u, x, ctx);
+	else if (cnt <= cap - part_room_pressed(cap) && under_pressure(H, st_home(s)))
+		try_list(H, sb, s, cap - part_room_pressed(cap), x, ctx);
 }
 
 __device__ __forceinline__ void undo(const kma_dheap *H, uint32_t sb, uint32_t m, uint32_t x, uint32_t ctx)
@@ -1016,14 +1054,19 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 				continue;
 			}
 			/* an empty superblock from the pool search, else a listed
-			 * partly free one of this XCD and class */
+			 * partly free one of this XCD and class; while the XCD is under
+			 * pressure (its last search found nothing) the listed ones come
+			 * first, the search when none is left, and the index once more
+			 * if the search found nothing either */
 			uint32_t sb, gen;
 			uint64_t old;
-			if (pool_pop(H, x, ctx, 0u, &sb)) {
+			const bool press = under_pressure(H, x);
+			bool listed = press && pidx_take(H, x, c, ctx, &sb);
+			if (!listed && pool_pop(H, x, ctx, 0u, &sb)) {
 				old = fadd(state(H, sb), ((uint64_t)(c + 1u) << ST_CLASS_SHIFT) | ST_OWNED |
 								 ((uint64_t)x << ST_HOME_SHIFT) | n);
 				gen = st_gen(old);
-			} else if (pidx_take(H, x, c, ctx, &sb)) {
+			} else if (listed || pidx_take(H, x, c, ctx, &sb)) {
 				old = attach_listed(H, sb, n);
 				gen = (st_gen(old) + 1u) & ST_GEN_MASK;
 				bump_stat(H, KMA_STAT_REATTACHED);

@@ -207,6 +207,74 @@ __device__ __forceinline__ uint32_t whole_superblocks(const kma_dheap *H, uint32
 	return tot * cap;
 }
 
+/* Partial grab (every lane of the wavefront calls; lanes of a group with
+ * need > 0 take part): under pool pressure a class still short of `need`
+ * blocks takes listed superblocks straight from the partial index, one per
+ * group lane (as many lanes as need / an eighth of a superblock, all in
+ * flight at once; taking more only returns them, and while they are out
+ * other callers find the index emptier).  The group prefix-sums their free slots and each lane
+ * takes only what the group still needs: one atomicAdd on the state word
+ * for the slots, then the lowest free bits of its bitmap words with one
+ * atomicOr per word.  Results go to res[order[0..done-1]] in lane order;
+ * returns the group's `done`.  Each superblock then goes back into the
+ * index or out of it (relist). */
+template <uint32_t G>
+__device__ inline uint32_t partial_superblocks(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c, uint32_t need,
+					       const uint16_t *order, uint64_t *res)
+{
+	const uint32_t gl = grp_lane<G>();
+	const uint32_t cap = need ? H->class_cap[c] : 32u, room = part_room(cap);
+	const uint32_t lanes = need ? min(G, (need + room - 1u) / room) : 0u;
+	uint32_t sb = 0;
+	const bool have = gl < lanes && pidx_take(H, x, c, ctx * 37u + gl * 11u, &sb);
+	const uint32_t fr = have ? sat_sub(cap, st_count(ld(state(H, sb)))) : 0u;
+	uint32_t tot;
+	const uint32_t pre = group_excl_scan<G>(fr, &tot);
+	const uint32_t take = have ? min(fr, sat_sub(need, pre)) : 0u;
+	uint32_t got = 0;
+	if (take) {
+		const uint64_t old = fadd(state(H, sb), (uint64_t)take);
+		got = min(take, sat_sub(cap, st_count(old)));
+		if (got < take)
+			undo(H, sb, take - got, x, ctx);
+		bump_stat(H, KMA_STAT_REATTACHED);
+	}
+	/* the lowest free bits of the superblock's words, up to `got` */
+	const uint32_t W = (cap + 31u) >> 5;
+	const uint32_t tail = (cap & 31u) ? ((1u << (cap & 31u)) - 1u) : ~0u;
+	uint32_t won[KMA_WHOLE_WORDS];
+	uint32_t nwon = 0;
+	uint32_t *bm = got ? bitmap(H, sb) : nullptr;
+#pragma unroll
+	for (uint32_t i = 0; i < KMA_WHOLE_WORDS; i++) {
+		won[i] = 0;
+		if (got > nwon && i < W) {
+			const uint32_t fb = ~ld(bm + i) & (i == W - 1u ? tail : ~0u);
+			const uint32_t pick = low_bits(fb, got - nwon);
+			if (pick) {
+				won[i] = pick & ~f_or(bm + i, pick);
+				nwon += __popc(won[i]);
+			}
+		}
+	}
+	if (got > nwon)
+		undo(H, sb, got - nwon, x, ctx); /* raced by a reservation still in flight */
+	uint32_t wtot;
+	uint32_t r = group_excl_scan<G>(nwon, &wtot);
+	if (nwon) {
+		const uint64_t d0 = H->off_data + ((uint64_t)sb << H->sb_log2);
+		const uint32_t bsize = H->class_size[c];
+#pragma unroll
+		for (uint32_t i = 0; i < KMA_WHOLE_WORDS; i++)
+			for (uint32_t m = won[i]; m; m &= m - 1u)
+				res[order[r++]] = d0 + (uint64_t)(i * 32u + (uint32_t)(__ffs(m) - 1)) * bsize;
+	}
+	if (have)
+		relist(H, sb, x, ctx, true);
+	return wtot;
+}
+#define KMA_PART_ROUNDS 4u
+
 __device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
 {
 	return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (n - 1u);
@@ -234,6 +302,7 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 	const uint32_t tid = threadIdx.x, nth = blockDim.x, nw = (nth + 63u) >> 6;
 	const uint32_t x = xcc(), ctx = wg_ctx_of(H, x);
 	const uint32_t ncls = H->nclasses;
+	const bool press = under_pressure(H, x);
 	if (tid < 32)
 		L.cnt[tid] = 0;
 	__syncthreads();
@@ -272,8 +341,19 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 		 * flight at once, and claims every slot with no reads */
 		const uint32_t cap = n ? H->class_cap[k] : 0u;
 		const uint32_t nwhole = (n && cap > 1u && cap <= 32u * KMA_WHOLE_WORDS) ? min(n / cap, G) : 0u;
-		if (__ballot(nwhole != 0u))
+		if (__ballot(nwhole != 0u) && !press)
 			r0 = whole_superblocks<G>(H, x, ctx, k, cap, gl < nwhole, ord, L.res);
+		/* under pool pressure the class takes listed superblocks instead,
+		 * several at once, before any head refill */
+		for (uint32_t rd = 0; press && rd < KMA_PART_ROUNDS; rd++) {
+			const uint32_t need = r0 < n && cap > 1u && cap <= 32u * KMA_WHOLE_WORDS ? n - r0 : 0u;
+			if (!__ballot(need != 0u))
+				break;
+			const uint32_t done = partial_superblocks<G>(H, x, ctx, k, need, ord + r0, L.res);
+			r0 += done;
+			if (!__ballot(need != 0u && done != 0u))
+				break;
+		}
 		bool pend = r0 < n;
 		waiter wt;
 		while (__ballot(pend)) {

@@ -89,7 +89,7 @@ typedef struct kma_run_stats {
 	uint64_t deadline_hits; /* reference layout: mallocs that returned NULL
 				   because deadline_ms had passed */
 	uint64_t listed;        /* native: detached superblocks put into a partial index */
-	uint64_t reattached;    /* native: listed superblocks attached to a head again */
+	uint64_t reattached;    /* native: listed superblocks reused (attached to a head again, or grabbed) */
 } kma_run_stats;
 
 /* Run a per-work-item workload on the heap. */

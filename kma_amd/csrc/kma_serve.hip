@@ -11,7 +11,9 @@
  *     arrival order (requeued jobs first) while a slot is free and the
  *     admission policy allows; decide whether the run is over;
  *   B (all work-groups): work-group g serves job slots [256 k, 256 k + 256)
- *     for k = g, g + G, ...: ONE collective kma_wg_malloc of block_bytes for
+ *     for k = (g + step) mod S, (g + G + step) mod S, ... (S slices of 256
+ *     slots; the rotation takes every job's blocks from every XCD's
+ *     superblocks alike): ONE collective kma_wg_malloc of block_bytes for
  *     every live slot, the block recorded in the slot's block table and
  *     stamped with pattern(job, step); a job that reaches its length appends
  *     (slot, job, first, count) to the done list, whose 64-bit counter keeps
@@ -585,7 +587,7 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 		init_slot_stacks(a, P);
 	const uint64_t t0 = ctl[C_T0];
 	uint64_t tA = wall_clock64();
-	for (;;) {
+	for (uint64_t lstep = 0;; lstep++) {
 		/* ---- A: retire, admit, stop? ---- */
 		if (wg0) {
 			__shared__ uint64_t s_pooled;
@@ -604,8 +606,12 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 		ph_a += tB - tA;
 		if (ctl[C_STOP])
 			break;
-		/* ---- B: grow every live job by one block ---- */
-		for (uint32_t sl = blockIdx.x; sl < nslices; sl += nwg) {
+		/* ---- B: grow every live job by one block.  Work-group j grows
+		 * slice (j + step) mod nslices (and j + nwg, ...): a slice moves to
+		 * the next work-group, hence to the next XCD, every step, so every
+		 * job takes its blocks from all XCDs' superblocks alike ---- */
+		for (uint32_t j = blockIdx.x; j < nslices; j += nwg) {
+			const uint32_t sl = (uint32_t)((j + lstep) % nslices);
 			const uint32_t s = sl * SV_BLOCK + tid;
 			const int64_t job0 = a.slot_job[s];
 			const uint32_t steps = job0 >= 0 ? a.slot_steps[s] : 0u;
