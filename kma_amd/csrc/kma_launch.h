@@ -103,8 +103,7 @@ typedef struct kma_serve_args {
 	int64_t *slot_job;        /* -1: free */
 	uint32_t *slot_steps;
 	uint64_t *table;          /* [max_live][max_len] block addresses */
-	uint32_t *free_slots;     /* one stack per XCD, max_live entries each */
-	uint32_t *wg_xcd;         /* per work-group: the XCD it runs on */
+	uint32_t *free_slots;     /* stack, max_live entries */
 	uint64_t *ctl;
 	uint64_t *done;           /* [max_live][4]: slot, job, first, count */
 	uint64_t *result;         /* per job: sum of its blocks' patterns */

@@ -168,7 +168,7 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	kma_serve_spec sp = *spec;
 	if (sp.max_admit == 0)
 		sp.max_admit = 1024;
-	if (sp.max_live == 0 || (sp.max_live & 255u) || sp.max_live > 1024u * 256u || sp.max_len == 0 || sp.block_bytes == 0 ||
+	if (sp.max_live == 0 || (sp.max_live & 255u) || sp.max_len == 0 || sp.block_bytes == 0 ||
 	    sp.watermark_pct == 0 || sp.watermark_pct > (sp.admit == KMA_SERVE_ADMIT_PROJECTED ? 400u : 100u) ||
 	    sp.duration_ms <= 0 || (!jobs && sp.rate_per_ms <= 0) || sp.admit > KMA_SERVE_ADMIT_WATERMARK) {
 		set_err("serving loop: bad spec");
@@ -236,19 +236,18 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	a.cap_blocks = cap_blocks;
 	a.keep_live = sp.keep_live != 0;
 	a.duration_ticks = (uint64_t)(sp.duration_ms * ticks_per_ms);
-	void *bufs[17] = {0};
+	void *bufs[16] = {0};
 	uint64_t nn = n ? n : 1, rq = (uint64_t)sp.max_live * 4;
-	/* [5]: one free-slot stack per XCD, filled on the device (the XCD of each
-	 * work-group is known only there); [16]: each work-group's XCD */
-	uint64_t sizes[17] = {nn * 8, nn * 4, (uint64_t)sp.max_live * 8, (uint64_t)sp.max_live * 4,
-			      (uint64_t)sp.max_live * sp.max_len * 8, (uint64_t)sp.max_live * 4 * 8, KMA_SV_CTL_WORDS * 8,
+	/* [5]: the free-slot stack, filled on the device */
+	uint64_t sizes[16] = {nn * 8, nn * 4, (uint64_t)sp.max_live * 8, (uint64_t)sp.max_live * 4,
+			      (uint64_t)sp.max_live * sp.max_len * 8, (uint64_t)sp.max_live * 4, KMA_SV_CTL_WORDS * 8,
 			      (uint64_t)sp.max_live * 32, nn * 8, nn * 8, nn * 8, KMA_SV_NHIST * 8,
-			      (uint64_t)sp.max_live * 4, (uint64_t)sp.max_live * 16, rq * 8, nn * 8, (uint64_t)nwg * 4};
+			      (uint64_t)sp.max_live * 4, (uint64_t)sp.max_live * 16, rq * 8, nn * 8};
 	int rc = -1;
 	hipStream_t st = NULL;
 	hipEvent_t e0 = NULL, e1 = NULL;
 	uint64_t *hist = NULL, *adm = NULL, *waits = NULL;
-	for (int i = 0; i < 17; i++)
+	for (int i = 0; i < 16; i++)
 		if ((i != 15 || jobs) && hipMalloc(&bufs[i], sizes[i]) != hipSuccess) {
 			set_err("serving loop: device allocation of %llu bytes failed", (unsigned long long)sizes[i]);
 			goto done;
@@ -271,7 +270,6 @@ int kma_serve_run(kma_heap *h, const kma_serve_spec *spec, const uint64_t *jobs,
 	a.requeue = (uint64_t *)bufs[14];
 	a.rq_cap = rq;
 	a.job_id = jobs ? (const uint64_t *)bufs[15] : NULL;
-	a.wg_xcd = (uint32_t *)bufs[16];
 	uint64_t ctl0[KMA_SV_CTL_WORDS];
 	memset(ctl0, 0, sizeof(ctl0));
 	if ((jobs && hipMemcpy(bufs[15], jobs, n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
@@ -398,7 +396,7 @@ done:
 		hipEventDestroy(e1);
 	if (st)
 		hipStreamDestroy(st);
-	for (int i = 0; i < 17; i++)
+	for (int i = 0; i < 16; i++)
 		if (bufs[i])
 			hipFree(bufs[i]);
 	return rc;

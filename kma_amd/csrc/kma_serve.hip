@@ -81,7 +81,7 @@ enum : uint32_t {
 	C_BAR_CNT = 0,
 	C_BAR_GEN = 16,
 	C_NEXT = 32,  /* next job to admit */
-	C_NFREE = 48, /* free slots over all XCDs' stacks (phase A keeps them per XCD in LDS) */
+	C_NFREE = 48, /* free slots (phase A keeps the stack top in LDS) */
 	C_STOP = 64,
 	C_DONE = 80, /* done list: entries << 40 | blocks */
 	C_T0 = 96,
@@ -131,7 +131,6 @@ constexpr uint64_t MASK40 = (1ull << 40) - 1ull;
 constexpr uint32_t SV_HYST_PCT = 10; /* admission hysteresis, points below the watermark */
 constexpr uint64_t BARRIER_TICKS = 500000000ull; /* 5 s */
 constexpr uint32_t SV_TL = KMA_SERVE_MAX_PROJECTED_LEN;
-constexpr uint32_t SV_MAX_SLICES = 1024; /* max_live / 256 */
 static_assert((SV_TL & (SV_TL - 1u)) == 0, "timeline length is a power of two");
 
 /* Work-group 0's LDS for phase A: the projected-use timeline and the values
@@ -139,14 +138,8 @@ static_assert((SV_TL & (SV_TL - 1u)) == 0, "timeline length is a power of two");
 struct sv_admit_lds {
 	uint32_t tl[SV_TL];
 	uint32_t part[SV_BLOCK];       /* ring_scan: per-thread chunk totals */
-	/* job slots: one free-slot stack per XCD (a.free_slots[x * max_live ..]),
-	 * the lengths of the jobs live on each XCD, and each slice's XCD (the
-	 * XCD of the work-group that grows it) */
-	uint32_t nfree_x[KMA_NXCD], committed[KMA_NXCD], add_x[KMA_NXCD];
-	uint32_t xoff[KMA_NXCD + 1];
-	uint8_t slice_xcd[SV_MAX_SLICES];
-	uint16_t xsl[SV_MAX_SLICES];
-	uint32_t n_rq, n_done; /* retirement: positions counted in LDS */
+	uint32_t nfree;                /* free-slot stack top (a.free_slots) */
+	uint32_t n_free, n_rq, n_done; /* retirement: positions counted in LDS */
 	uint32_t rebuild, cand_len;
 	int64_t cand;
 	uint64_t now, step;
@@ -239,50 +232,18 @@ __device__ __forceinline__ void ring_scan(sv_admit_lds &P, uint64_t start)
 	__syncthreads();
 }
 
-/* Thread 0 of work-group 0: a free slot on the XCD whose live jobs have
- * the smallest total length (the heap's superblocks serve the growth of
- * their own XCD, so admissions spread the demand evenly over them); its
- * total grows by len.  UINT32_MAX if no slot is free. */
-__device__ __forceinline__ uint32_t pop_slot(const kma_serve_args &a, sv_admit_lds &P, uint32_t len)
+/* Work-group 0, all threads, once: the free-slot stack holds every slot,
+ * ordered so that successive pops cycle over the slices (pop k: slice
+ * k mod S, lane k / S), which spreads the live jobs evenly over the slices
+ * and so over the work-groups that grow them; a retired slot goes back on
+ * top, so the next admission refills the same slice. */
+__device__ inline void init_slot_stack(const kma_serve_args &a, sv_admit_lds &P)
 {
-	int bx = -1;
-	for (int x = 0; x < (int)KMA_NXCD; x++)
-		if (P.nfree_x[x] && (bx < 0 || P.committed[x] < P.committed[bx]))
-			bx = x;
-	if (bx < 0)
-		return UINT32_MAX;
-	P.committed[bx] += len;
-	return a.free_slots[(uint64_t)bx * a.max_live + --P.nfree_x[bx]];
-}
-
-/* Work-group 0, all threads, once after every group has written its XCD:
- * each XCD's stack holds the slots of its slices, ordered so that its pops
- * cycle over those slices (pop j: slice j mod n, lane j / n). */
-__device__ inline void init_slot_stacks(const kma_serve_args &a, sv_admit_lds &P)
-{
-	const uint32_t tid = threadIdx.x, nslices = a.max_live / SV_BLOCK;
-	if (tid == 0) {
-		uint32_t cnt[KMA_NXCD] = {0, 0, 0, 0, 0, 0, 0, 0};
-		for (uint32_t sl = 0; sl < nslices; sl++) {
-			P.slice_xcd[sl] = (uint8_t)(a.wg_xcd[sl % gridDim.x] & (KMA_NXCD - 1u));
-			cnt[P.slice_xcd[sl]]++;
-		}
-		P.xoff[0] = 0;
-		for (uint32_t x = 0; x < KMA_NXCD; x++) {
-			P.xoff[x + 1] = P.xoff[x] + cnt[x];
-			P.nfree_x[x] = cnt[x] * SV_BLOCK;
-			P.committed[x] = 0;
-			cnt[x] = 0;
-		}
-		for (uint32_t sl = 0; sl < nslices; sl++)
-			P.xsl[P.xoff[P.slice_xcd[sl]] + cnt[P.slice_xcd[sl]]++] = (uint16_t)sl;
-	}
-	__syncthreads();
-	for (uint32_t x = 0; x < KMA_NXCD; x++) {
-		const uint32_t n = P.xoff[x + 1] - P.xoff[x], cap = n * SV_BLOCK;
-		for (uint32_t j = tid; j < cap; j += SV_BLOCK)
-			a.free_slots[(uint64_t)x * a.max_live + cap - 1u - j] = P.xsl[P.xoff[x] + j % n] * SV_BLOCK + j / n;
-	}
+	const uint32_t S = a.max_live / SV_BLOCK;
+	for (uint32_t k = threadIdx.x; k < a.max_live; k += SV_BLOCK)
+		a.free_slots[a.max_live - 1u - k] = (k % S) * SV_BLOCK + k / S;
+	if (threadIdx.x == 0)
+		P.nfree = a.max_live;
 	__syncthreads();
 }
 
@@ -310,35 +271,31 @@ __device__ __forceinline__ void phase_a(const kma_serve_args &a, uint64_t t0, ui
 		dc = ctl[C_DONE];
 		P.now = wall_clock64() - t0;
 		P.step = S[S_STEPS];
-		P.n_rq = P.n_done = 0;
+		P.n_free = P.n_rq = P.n_done = 0;
 	}
-	if (tid < KMA_NXCD)
-		P.add_x[tid] = 0;
 	__syncthreads();
 	const uint64_t now = P.now, step = P.step;
 	/* ---- retire the previous step's entries: completions and preemptions,
-	 * one entry per thread; the slot goes back on its XCD's stack ---- */
+	 * one entry per thread; the slot goes back on the free-slot stack ---- */
 	{
 		const uint32_t ne = (uint32_t)(ctl[C_DONE] >> 40);
 		const uint64_t rqt0 = ctl[C_RQ_TAIL];
 		for (uint32_t e = tid; e < ne; e += SV_BLOCK) {
 			const uint64_t sf = a.done[4u * e], job = a.done[4u * e + 1u];
-			const uint32_t slot = (uint32_t)(sf & ~PFLAG), x = P.slice_xcd[slot / SV_BLOCK];
+			const uint32_t slot = (uint32_t)(sf & ~PFLAG);
 			if (sf & PFLAG) {
 				a.requeue[(rqt0 + atomicAdd(&P.n_rq, 1u)) % a.rq_cap] = job;
 			} else {
 				a.job_done[job] = now + 1u;
 				a.slot_job[slot] = -1;
 				atomicAdd(&P.n_done, 1u);
-				atomicSub(&P.committed[x], a.length[job]);
 			}
-			a.free_slots[(uint64_t)x * a.max_live + P.nfree_x[x] + atomicAdd(&P.add_x[x], 1u)] = slot;
+			a.free_slots[P.nfree + atomicAdd(&P.n_free, 1u)] = slot;
 		}
 	}
 	__syncthreads();
-	if (tid < KMA_NXCD)
-		P.nfree_x[tid] += P.add_x[tid];
 	if (tid == 0) {
+		P.nfree += P.n_free;
 		rqt += P.n_rq;
 		live -= P.n_done;
 		S[S_DONE] += P.n_done;
@@ -393,7 +350,6 @@ __device__ __forceinline__ void phase_a(const kma_serve_args &a, uint64_t t0, ui
 				a.done[4u * k + 2u] = dc & MASK40;
 				a.done[4u * k + 3u] = blocks;
 				dc += ((uint64_t)1 << 40) + blocks;
-				P.committed[P.slice_xcd[slot / SV_BLOCK]] -= a.length[a.slot_job[slot]];
 				a.slot_job[slot] = -1;
 				live--;
 				S[S_PREEMPT]++;
@@ -440,10 +396,7 @@ __device__ __forceinline__ void phase_a(const kma_serve_args &a, uint64_t t0, ui
 		for (;;) {
 			if (tid == 0) {
 				int64_t c = -1;
-				uint32_t nfree = 0;
-				for (uint32_t x = 0; x < KMA_NXCD; x++)
-					nfree += P.nfree_x[x];
-				if (!stop && !hold && nfree > 0 && nadm < a.max_admit) {
+				if (!stop && !hold && P.nfree > 0 && nadm < a.max_admit) {
 					if (rqh != rqt)
 						c = (int64_t)a.requeue[rqh % a.rq_cap];
 					else if (next < a.njobs && a.arrival[next] <= now)
@@ -471,7 +424,7 @@ __device__ __forceinline__ void phase_a(const kma_serve_args &a, uint64_t t0, ui
 					next++;
 					a.job_admit[job] = now + 1u;
 				}
-				const uint32_t slot = pop_slot(a, P, len);
+				const uint32_t slot = a.free_slots[--P.nfree];
 				a.slot_job[slot] = job;
 				a.slot_steps[slot] = 0;
 				a.slot_seq[slot] = (uint32_t)++seq;
@@ -489,24 +442,17 @@ __device__ __forceinline__ void phase_a(const kma_serve_args &a, uint64_t t0, ui
 			}
 		}
 	} else if (tid == 0 && !stop && !hold) {
-		while (nadm < a.max_admit && used * 100u < (uint64_t)a.watermark_pct * H->nsb) {
+		while (P.nfree > 0 && nadm < a.max_admit && used * 100u < (uint64_t)a.watermark_pct * H->nsb) {
 			uint64_t job;
 			if (rqh != rqt) {
-				job = a.requeue[rqh % a.rq_cap];
+				job = a.requeue[rqh++ % a.rq_cap];
 			} else if (next < a.njobs && a.arrival[next] <= now) {
-				job = next;
-			} else {
-				break;
-			}
-			const uint32_t slot = pop_slot(a, P, a.length[job]);
-			if (slot == UINT32_MAX)
-				break;
-			if (rqh != rqt) {
-				rqh++;
-			} else {
-				next++;
+				job = next++;
 				a.job_admit[job] = now + 1u;
+			} else {
+				break;
 			}
+			const uint32_t slot = a.free_slots[--P.nfree];
 			a.slot_job[slot] = (int64_t)job;
 			a.slot_steps[slot] = 0;
 			a.slot_seq[slot] = (uint32_t)++seq;
@@ -536,10 +482,7 @@ __device__ __forceinline__ void phase_a(const kma_serve_args &a, uint64_t t0, ui
 		} else {
 			S[S_RUN_TICKS] = now;
 		}
-		uint32_t nfree = 0;
-		for (uint32_t x = 0; x < KMA_NXCD; x++)
-			nfree += P.nfree_x[x];
-		ctl[C_NFREE] = nfree;
+		ctl[C_NFREE] = P.nfree;
 		ctl[C_LIVE] = live;
 		ctl[C_NEXT] = next;
 		ctl[C_LTOP] = ltop;
@@ -569,8 +512,6 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 	uint64_t ph_awork = 0, ph_a = 0, ph_b = 0, ph_c = 0, w_b = 0, w_c = 0;
 	if (wg0 && tid == 0)
 		ctl[C_T0] = wall_clock64();
-	if (tid == 0)
-		a.wg_xcd[blockIdx.x] = xcc();
 	if (wg0)
 		for (uint32_t i = tid; i < SV_TL; i += SV_BLOCK)
 			P.tl[i] = 0u;
@@ -584,7 +525,7 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 	if (!grid_sync(ctl, nwg))
 		return;
 	if (wg0)
-		init_slot_stacks(a, P);
+		init_slot_stack(a, P);
 	const uint64_t t0 = ctl[C_T0];
 	uint64_t tA = wall_clock64();
 	for (uint64_t lstep = 0;; lstep++) {
