@@ -62,6 +62,41 @@ int64_t kma_queue_items(const kma_queue *q, uint64_t *items, uint64_t max);
 int kma_queue_run(kma_queue *q, uint32_t mode, uint64_t nwi, uint32_t rounds, uint64_t *base_out,
 		  int64_t *order_out);
 
+/* ---- unbounded linked pointer queue ---- */
+typedef struct kma_lqueue kma_lqueue;
+
+/* clQueue_create(dev, ctx, cq, prg), clQueue.h:68, with the reference's
+ * contract: items are linked through their first 8 bytes
+ * (clqueue_item.next, clQueue.h:42-45), so the queue holds any number of
+ * them and enqueue never fails for capacity (kma_lqueue.h). */
+kma_lqueue *kma_lqueue_create(void);
+void kma_lqueue_destroy(kma_lqueue *q);
+/* Device address to pass to kernels (kma_dlqueue *, kma_lqueue.h). */
+void *kma_lqueue_device(const kma_lqueue *q);
+/* Empty the queue (clQueue_init, clQueue.cl:27-32). */
+int kma_lqueue_reset(kma_lqueue *q);
+/* The reference's validator (tb_clQueue.c:126-157) on the chain from head:
+ * it must end (no cycle), its last node must be the one tail names, and
+ * every node must be an item of the runner's array or the queue's own stub,
+ * each at most once. */
+typedef struct kma_lqueue_report {
+	uint64_t items;    /* items on the chain (the stub not counted) */
+	uint64_t tail_ok;  /* 1: the chain ends at the node tail names */
+	uint64_t cycle;    /* 1: the chain did not end within items + 1 nodes */
+	uint64_t stubs;    /* times the chain passes the stub (0 or 1) */
+	uint64_t outside;  /* links to neither an item nor the stub */
+	uint64_t repeats;  /* items met twice */
+	uint64_t lost;     /* enqueue or dequeue calls that gave up (corrupt queue) */
+} kma_lqueue_report;
+/* Test runner on a runner-owned item array of nwi 16-byte items (the modes
+ * of kma_queue_run): 0 every work-item enqueues its item; 1 then `rounds`
+ * times dequeues one item and enqueues it again; 2 one work-item enqueues
+ * items 0..nwi-1 and dequeues until empty, order_out[k] = index of the k-th
+ * item dequeued (-1 past the end).  The chain is validated after the
+ * kernels (*rep). */
+int kma_lqueue_run(kma_lqueue *q, uint32_t mode, uint64_t nwi, uint32_t rounds, int64_t *order_out,
+		   kma_lqueue_report *rep);
+
 /* ---- array list ---- */
 typedef struct kma_list kma_list;
 

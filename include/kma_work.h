@@ -369,7 +369,7 @@ int kma_serve_lengths(const kma_serve_spec *spec, const uint64_t *jobs, uint64_t
  * Every edge {source, sink} makes sure both node ids are in an insert-only
  * tree (tb_clTree.cl:12-38 clGraph_node_ensure), allocates a link and
  * appends it to the source node's adjacency list (tb_clTree.cl:57-67).
- * Nodes (32 B) and links (16 B) come from
+ * Nodes (48 B: a tree node and the queue of its links) and links (16 B) come from
  *   KMA_GRAPH_PLAIN       one kma_malloc per lane; a node that lost its
  *                         insert race is freed (clTree_test,
  *                         tb_clTree.cl:41-85);

@@ -38,6 +38,7 @@ EXPORTS = [
     "kma_batch_malloc_det", "kma_batch_free_det", "kma_serve_run", "kma_serve_trace", "kma_serve_lengths",
     "kma_graph_generate", "kma_graph_build", "kma_image_check", "kma_heap_check",
     "kma_batch_malloc_det_noalloc", "kma_det_noalloc_scratch_bytes",
+    "kma_lqueue_create", "kma_lqueue_destroy", "kma_lqueue_device", "kma_lqueue_reset", "kma_lqueue_run",
 ]
 DET_NOALLOC_MAX = 524288
 GRAPH_PLAIN, GRAPH_CACHE, GRAPH_COLLECTIVE = 0, 1, 2
@@ -308,6 +309,14 @@ def lib():
         L.kma_queue_items.argtypes = [vp, vp, u64]
         L.kma_queue_items.restype = i64
         L.kma_queue_run.argtypes = [vp, u32, u64, u32, P(u64), vp]
+        L.kma_lqueue_create.argtypes = []
+        L.kma_lqueue_create.restype = vp
+        L.kma_lqueue_destroy.argtypes = [vp]
+        L.kma_lqueue_destroy.restype = None
+        L.kma_lqueue_device.argtypes = [vp]
+        L.kma_lqueue_device.restype = vp
+        L.kma_lqueue_reset.argtypes = [vp]
+        L.kma_lqueue_run.argtypes = [vp, u32, u64, u32, vp, P(LQueueReport)]
         L.kma_list_create.argtypes = [vp, u32, u64]
         L.kma_list_create.restype = vp
         L.kma_list_destroy.argtypes = [vp]
@@ -676,6 +685,35 @@ class Queue(_Handle):
         _check(lib().kma_queue_run(self._p, mode, nwi, rounds, ctypes.byref(base),
                                    order.ctypes.data if order is not None else None), "kma_queue_run")
         return base.value, (order[:nwi] if order is not None else None)
+
+
+class LQueueReport(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("items", "tail_ok", "cycle", "stubs", "outside", "repeats", "lost")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LinkedQueue(_Handle):
+    """Unbounded intrusive MPMC queue (kma_lqueue_create; the reference's
+    clQueue_create with its contract: items linked through their first word)."""
+    _destroy = "kma_lqueue_destroy"
+
+    def __init__(self):
+        self._p = lib().kma_lqueue_create()
+        if not self._p:
+            raise KmaError("kma_lqueue_create: " + last_error())
+
+    def reset(self):
+        _check(lib().kma_lqueue_reset(self._p), "kma_lqueue_reset")
+
+    def run(self, mode, nwi, rounds=0):
+        """(report dict, order array for mode 2 or None)."""
+        rep = LQueueReport()
+        order = np.zeros(max(nwi, 1), dtype=np.int64) if mode == 2 else None
+        _check(lib().kma_lqueue_run(self._p, mode, nwi, rounds, order.ctypes.data if order is not None else None,
+                                    ctypes.byref(rep)), "kma_lqueue_run")
+        return rep.as_dict(), (order[:nwi] if order is not None else None)
 
 
 class ArrayList(_Handle):

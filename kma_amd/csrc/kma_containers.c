@@ -263,6 +263,149 @@ done:
 	return rc;
 }
 
+/* ---------------- linked pointer queue ---------------- */
+/* header words (kma_dlqueue in kma_lqueue.h) */
+#define LQ_BYTES 512u
+#define LQ_HEAD 16u
+#define LQ_TAIL 32u
+#define LQ_STUB 48u
+#define LQ_PTR ((1ull << 48) - 1ull)
+
+struct kma_lqueue {
+	char *dev;
+};
+
+kma_lqueue *kma_lqueue_create(void)
+{
+	if (kma_device_ok() != 0)
+		return NULL;
+	kma_lqueue *q = (kma_lqueue *)calloc(1, sizeof(*q));
+	if (!q)
+		return NULL;
+	if (hipMalloc((void **)&q->dev, LQ_BYTES) != hipSuccess) {
+		set_err("linked queue: device allocation failed");
+		free(q);
+		return NULL;
+	}
+	if (((uint64_t)(uintptr_t)q->dev + LQ_BYTES) & ~LQ_PTR) {
+		set_err("linked queue: device address above 2^48");
+		hipFree(q->dev);
+		free(q);
+		return NULL;
+	}
+	if (kma_lqueue_reset(q) != 0) {
+		hipFree(q->dev);
+		free(q);
+		return NULL;
+	}
+	return q;
+}
+
+void kma_lqueue_destroy(kma_lqueue *q)
+{
+	if (!q)
+		return;
+	hipFree(q->dev);
+	free(q);
+}
+
+void *kma_lqueue_device(const kma_lqueue *q)
+{
+	return q ? q->dev : NULL;
+}
+
+int kma_lqueue_reset(kma_lqueue *q)
+{
+	if (!q)
+		return -1;
+	HIPCHK(hipMemset(q->dev, 0, LQ_BYTES), -1);
+	LCHK(kmad_lqueue_init(q->dev), -1);
+	return 0;
+}
+
+/* Walk the chain from head over host copies of the header and the items. */
+static void lqueue_walk(const uint64_t *hdr, uint64_t dev_hdr, const uint64_t *items, uint64_t dev_items,
+			uint64_t n, kma_lqueue_report *r)
+{
+	uint8_t *seen = (uint8_t *)calloc(n ? n : 1, 1);
+	const uint64_t stub = dev_hdr + 8u * LQ_STUB, tail = hdr[LQ_TAIL] & LQ_PTR;
+	uint64_t cur = hdr[LQ_HEAD] & LQ_PTR, last = 0, steps = 0;
+	while (cur && steps <= n + 1u) {
+		uint64_t link;
+		if (cur == stub) {
+			r->stubs++;
+			link = hdr[LQ_STUB];
+		} else if (cur >= dev_items && cur < dev_items + 16u * n && (cur - dev_items) % 16u == 0) {
+			uint64_t i = (cur - dev_items) / 16u;
+			if (seen && seen[i]++)
+				r->repeats++;
+			r->items++;
+			link = items[2u * i];
+		} else {
+			r->outside++;
+			break;
+		}
+		last = cur;
+		cur = link & LQ_PTR;
+		steps++;
+	}
+	r->cycle = cur != 0 && r->outside == 0;
+	r->tail_ok = !r->cycle && !r->outside && last == tail;
+	free(seen);
+}
+
+int kma_lqueue_run(kma_lqueue *q, uint32_t mode, uint64_t nwi, uint32_t rounds, int64_t *order_out,
+		   kma_lqueue_report *rep)
+{
+	if (!q || mode > 2 || (mode == 2 && !order_out) || !rep) {
+		set_err("bad linked-queue run (mode %u)", mode);
+		return -1;
+	}
+	memset(rep, 0, sizeof(*rep));
+	char *items = NULL;
+	int64_t *dorder = NULL;
+	unsigned long long *dlost = NULL, lost = 0;
+	uint64_t *hitems = NULL, hdr[LQ_BYTES / 8];
+	int rc = -1;
+	DEV_ALLOC(items, (nwi ? nwi : 1) * 16, goto done);
+	DEV_ALLOC(dlost, 8, goto done);
+	if (mode == 2)
+		DEV_ALLOC(dorder, (nwi ? nwi : 1) * 8, goto done);
+	if (((uint64_t)(uintptr_t)items + nwi * 16) & ~LQ_PTR) {
+		set_err("linked queue: item address above 2^48");
+		goto done;
+	}
+	if (hipMemset(dlost, 0, 8) != hipSuccess ||
+	    kmad_lqueue_run(q->dev, mode, items, nwi, rounds, dorder, dlost) != 0) {
+		set_err("linked-queue kernels failed");
+		goto done;
+	}
+	hitems = (uint64_t *)malloc((nwi ? nwi : 1) * 16);
+	if (!hitems || hipMemcpy(hitems, items, nwi * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+	    hipMemcpy(hdr, q->dev, LQ_BYTES, hipMemcpyDeviceToHost) != hipSuccess ||
+	    hipMemcpy(&lost, dlost, 8, hipMemcpyDeviceToHost) != hipSuccess ||
+	    (mode == 2 && hipMemcpy(order_out, dorder, nwi * 8, hipMemcpyDeviceToHost) != hipSuccess)) {
+		set_err("download failed");
+		goto done;
+	}
+	lqueue_walk(hdr, (uint64_t)(uintptr_t)q->dev, hitems, (uint64_t)(uintptr_t)items, nwi, rep);
+	rep->lost = lost;
+	rc = 0;
+done:
+	/* the items leave with this run: empty the queue so that no link to
+	 * them survives */
+	if (items && kma_lqueue_reset(q) != 0)
+		rc = -1;
+	free(hitems);
+	if (items)
+		hipFree(items);
+	if (dlost)
+		hipFree(dlost);
+	if (dorder)
+		hipFree(dorder);
+	return rc;
+}
+
 /* ---------------- array list ---------------- */
 kma_list *kma_list_create(kma_heap *heap, uint32_t obj_size, uint64_t max_pages)
 {

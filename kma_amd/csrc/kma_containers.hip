@@ -10,6 +10,7 @@
 #include "kma_launch.h"
 #include "kma_list.h"
 #include "kma_pma.h"
+#include "kma_lqueue.h"
 #include "kma_queue.h"
 #include "kma_tree.h"
 
@@ -150,6 +151,84 @@ extern "C" int kmad_queue_run(void *q, uint32_t mode, char *items, uint64_t n, u
 				   rounds);
 	else
 		hipLaunchKernelGGL(k_queue_serial, dim3(1), dim3(64), 0, 0, (kma_dqueue *)q, items, n, order);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}
+
+/* ---------------- linked pointer queue ---------------- */
+__global__ void k_lqueue_init(kma_dlqueue *q)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		q->magic = KMA_LQUEUE_MAGIC;
+		q->stub = 0;
+		q->head = (uint64_t)(uintptr_t)&q->stub;
+		q->tail = (uint64_t)(uintptr_t)&q->stub;
+	}
+}
+
+__global__ void k_lqueue_enqueue(kma_dlqueue *q, char *items, uint64_t n, unsigned long long *lost)
+{
+	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && !kma_lq_enqueue(q, items + 16u * i))
+		atomicAdd(lost, 1ull);
+}
+
+__global__ void k_lqueue_shuffle(kma_dlqueue *q, char *items, uint64_t n, uint32_t rounds, unsigned long long *lost)
+{
+	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	if (!kma_lq_enqueue(q, items + 16u * i))
+		atomicAdd(lost, 1ull);
+	for (uint32_t r = 0; r < rounds; r++) {
+		/* every lane calls (a NULL item is a no-op), as the wavefront
+		 * aggregation serves the lanes that call together */
+		void *it = kma_lq_dequeue(q);
+		if (!kma_lq_enqueue(q, it) && it)
+			atomicAdd(lost, 1ull);
+	}
+}
+
+__global__ void k_lqueue_serial(kma_dlqueue *q, char *items, uint64_t n, int64_t *order, unsigned long long *lost)
+{
+	if (blockIdx.x != 0 || threadIdx.x != 0)
+		return;
+	for (uint64_t i = 0; i < n; i++)
+		if (!kma_lq_enqueue(q, items + 16u * i))
+			atomicAdd(lost, 1ull);
+	uint64_t k = 0;
+	for (;;) {
+		char *it = (char *)kma_lq_dequeue(q);
+		if (!it)
+			break;
+		if (k < n)
+			order[k] = (int64_t)((it - items) / 16);
+		k++;
+	}
+	for (; k < n; k++)
+		order[k] = -1;
+}
+
+extern "C" int kmad_lqueue_init(void *q)
+{
+	hipLaunchKernelGGL(k_lqueue_init, dim3(1), dim3(64), 0, 0, (kma_dlqueue *)q);
+	KMA_CHECK(hipGetLastError());
+	KMA_CHECK(hipDeviceSynchronize());
+	return 0;
+}
+
+extern "C" int kmad_lqueue_run(void *q, uint32_t mode, char *items, uint64_t n, uint32_t rounds, int64_t *order,
+			       unsigned long long *lost)
+{
+	if (mode == 0)
+		hipLaunchKernelGGL(k_lqueue_enqueue, dim3(nblocks(n, 256)), dim3(256), 0, 0, (kma_dlqueue *)q, items, n,
+				   lost);
+	else if (mode == 1)
+		hipLaunchKernelGGL(k_lqueue_shuffle, dim3(nblocks(n, 256)), dim3(256), 0, 0, (kma_dlqueue *)q, items, n,
+				   rounds, lost);
+	else
+		hipLaunchKernelGGL(k_lqueue_serial, dim3(1), dim3(64), 0, 0, (kma_dlqueue *)q, items, n, order, lost);
 	KMA_CHECK(hipGetLastError());
 	KMA_CHECK(hipDeviceSynchronize());
 	return 0;

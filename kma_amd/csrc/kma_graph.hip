@@ -12,13 +12,16 @@
  *   - work-group collective kma_wg_malloc calls: one for the source nodes
  *     missing from the tree, one for the sink nodes, one for the links
  *     (tb_clTree.cl:175-273, clTree_test_al, which slices ArrayList pages).
- * Adjacency lists are lock-free stacks of links (the reference appends to a
- * per-node clQueue; the order of a node's links carries no meaning).
+ * Every node carries its adjacency list as an embedded linked queue
+ * (kma_lq, kma_lqueue.h: the reference's per-node clqueue that every link is
+ * enqueued to, tb_clTree.cl:67, 168, 271); a link's first word is its queue
+ * link.
  */
 #include <hip/hip_runtime.h>
 
 #include "kma_device.h"
 #include "kma_launch.h"
+#include "kma_lqueue.h"
 #include "kma_pma.h"
 #include "kma_tree.h"
 #include "kma_wg_device.h"
@@ -34,16 +37,16 @@ namespace {
 
 constexpr uint32_t GBLOCK = 256;
 
-/* A graph node is a tree node followed by the head of its adjacency list. */
+/* A graph node is a tree node followed by the queue of its links. */
 struct gnode {
 	kma_tree_node t;
-	uint64_t adj; /* glink * */
+	kma_lq adj; /* of glink */
 };
 struct glink {
-	uint64_t next; /* glink * */
+	uint64_t next; /* queue link (kma_lqueue.h) */
 	uint64_t sink; /* gnode * */
 };
-static_assert(sizeof(gnode) == 32 && sizeof(glink) == 16, "graph record sizes");
+static_assert(sizeof(gnode) == 48 && sizeof(glink) == 16, "graph record sizes");
 
 /* counters at stats[k], summed per wavefront */
 enum : uint32_t { G_EDGES, G_NODE_MALLOC, G_NODE_FREE, G_LINK_MALLOC, G_NULLS, G_BAD, G_N };
@@ -72,19 +75,30 @@ __device__ __forceinline__ void st64(uint64_t *p, uint64_t v)
 	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* Push link l (to sink) onto node src's adjacency list. */
-__device__ __forceinline__ void adj_push(gnode *src, glink *l, gnode *sink)
+/* Enqueue link l (to sink) on node src's adjacency queue; false if the
+ * queue refused it (a corrupt queue). */
+__device__ __forceinline__ bool adj_push(gnode *src, glink *l, gnode *sink)
 {
 	st64(&l->sink, (uint64_t)(uintptr_t)sink);
-	uint64_t h = __hip_atomic_load(&src->adj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	for (;;) {
-		st64(&l->next, h);
-		uint64_t seen = h;
-		if (__hip_atomic_compare_exchange_strong(&src->adj, &seen, (uint64_t)(uintptr_t)l, __ATOMIC_RELAXED,
-							 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-			return;
-		h = seen;
+	return kma_lq_push(&src->adj, l) != 0;
+}
+
+/* The links of a node no call is using, in queue order: from the node at
+ * the head, the queue's stub skipped; at most `limit` of them. */
+template <class F>
+__device__ __forceinline__ uint32_t adj_walk(gnode *nd, uint32_t limit, F &&f)
+{
+	uint32_t d = 0;
+	const uint64_t stub = (uint64_t)(uintptr_t)&nd->adj.stub;
+	for (uint64_t w = kma_lq_ptr(nd->adj.head); w && d <= limit;) {
+		const uint64_t nx = kma_lq_ptr(*(const uint64_t *)(uintptr_t)w);
+		if (w != stub) {
+			f((glink *)(uintptr_t)w);
+			d++;
+		}
+		w = nx;
 	}
+	return d;
 }
 
 /* The node with `key`: found in the tree, or `cache` inserted.  On a lost
@@ -99,7 +113,7 @@ __device__ __forceinline__ gnode *ensure_from(kma_tree *t, uint32_t key, gnode *
 			return n;
 		if (!cache)
 			return nullptr;
-		st64(&cache->adj, 0);
+		kma_lq_init(&cache->adj);
 		cache->t.key = key;
 		if (kma_tree_add(t, &cache->t)) {
 			n = cache;
@@ -146,7 +160,7 @@ __device__ __forceinline__ ensured ensure_lane(kma_tree *t, uint32_t key, gnode 
 			if (!cache)
 				return ensured{nullptr, mallocs, 1u, frees, 0u};
 		}
-		st64(&cache->adj, 0);
+		kma_lq_init(&cache->adj);
 		cache->t.key = key;
 		if (kma_tree_add(t, &cache->t)) {
 			n = cache;
@@ -196,8 +210,10 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_lane(kma_dheap *H, kma_pma_dhe
 		c.nulls += l ? 0u : 1u;
 		if (!l)
 			break;
-		adj_push(src, l, snk);
-		c.edges++;
+		if (adj_push(src, l, snk))
+			c.edges++;
+		else
+			c.bad++;
 	}
 	if (cache) {
 		a.put(cache);
@@ -244,10 +260,10 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_wg(kma_dheap *H, kma_tree *t, 
 		c.bad += lost;
 		const bool linkit = has && src && snk;
 		glink *l = (glink *)kma_wg_malloc<GBLOCK, 16u>(H, linkit ? sizeof(glink) : 0u, L);
-		if (linkit && l)
-			adj_push(src, l, snk);
+		const bool pushed = linkit && l && adj_push(src, l, snk);
 		c.link_malloc += linkit ? 1u : 0u;
-		c.edges += linkit && l ? 1u : 0u;
+		c.edges += pushed ? 1u : 0u;
+		c.bad += linkit && l && !pushed ? 1u : 0u;
 		c.nulls += linkit && !l ? 1u : 0u;
 		dead = dead || (i < n && !(linkit && l));
 	}
@@ -272,15 +288,24 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_check(kma_tree *t, const uint3
 		if (!nd) {
 			bad = 1;
 		} else {
-			uint64_t s = 0;
-			uint32_t d = 0;
-			for (uint64_t l = nd->adj; l && d <= deg[i]; d++) {
-				const glink *lk = (const glink *)(uintptr_t)l;
-				s += ((const gnode *)(uintptr_t)lk->sink)->t.key;
-				l = lk->next;
+			/* the reference's queue validator (tb_clQueue.c:126-157) on the
+			 * node's queue: the chain from head ends, at the node tail
+			 * names, passing the stub once; then the links' sinks */
+			uint64_t s = 0, last = 0, w = kma_lq_ptr(nd->adj.head);
+			const uint64_t stub = (uint64_t)(uintptr_t)&nd->adj.stub;
+			uint32_t d = 0, stubs = 0;
+			while (w && d <= deg[i] && stubs <= 1u) {
+				if (w == stub) {
+					stubs++;
+				} else {
+					s += ((const gnode *)(uintptr_t)((const glink *)(uintptr_t)w)->sink)->t.key;
+					d++;
+				}
+				last = w;
+				w = kma_lq_ptr(*(const uint64_t *)(uintptr_t)w);
 			}
 			links = d;
-			bad = d != deg[i] || s != ssum[i];
+			bad = w != 0 || stubs != 1u || last != kma_lq_ptr(nd->adj.tail) || d != deg[i] || s != ssum[i];
 		}
 	}
 	uint64_t x = bad, y = links;
@@ -305,12 +330,7 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_free(kma_dheap *H, const uint6
 	if (i >= nk || !nodes[i])
 		return;
 	gnode *nd = (gnode *)(uintptr_t)nodes[i];
-	uint32_t d = 0;
-	for (uint64_t l = nd->adj; l && d <= deg[i]; d++) {
-		glink *lk = (glink *)(uintptr_t)l;
-		l = lk->next;
-		kma_free(H, lk);
-	}
+	adj_walk(nd, deg[i], [&](glink *lk) { kma_free(H, lk); });
 	kma_free(H, nd);
 }
 

@@ -67,6 +67,9 @@ int kmad_stream_persistent(void *heap, const uint32_t *d_req, uint64_t *d_res, u
 int kmad_pma_init(void *h, uint64_t bytes, uint64_t align);
 int kmad_pma_run(void *h, const uint32_t *d_sizes, uint64_t n, int serial, uint64_t *d_out);
 int kmad_queue_init(void *q, uint64_t cap);
+int kmad_lqueue_init(void *q);
+int kmad_lqueue_run(void *q, uint32_t mode, char *d_items, uint64_t n, uint32_t rounds, int64_t *d_order,
+		    unsigned long long *d_lost);
 int kmad_queue_run(void *q, uint32_t mode, char *d_items, uint64_t n, uint32_t rounds, int64_t *d_order);
 int kmad_list_grow(void *l, const uint32_t *d_objs, uint64_t n, uint32_t block, int local, uint64_t *d_out);
 int kmad_list_get(void *l, const uint64_t *d_idx, uint64_t n, uint64_t *d_out);

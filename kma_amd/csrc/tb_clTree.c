@@ -143,10 +143,11 @@ static int run_form(unsigned f, const uint32_t *edges, uint64_t n)
 		printf("%s\n", forms[f].section);
 	printf("-- Executing %s --\n", forms[f].name);
 	uint64_t *us = (uint64_t *)calloc(samples, sizeof(uint64_t));
-	/* nodes <= 2 edges: at most 32 B per node and 16 B per edge live, plus
-	 * one cached 32-B node per lane; the heap adds a superblock per context
-	 * for each of the two classes (nodes, links) */
-	uint64_t need = 80ull * n;
+	/* nodes <= 2 edges: at most 64 B per node (48-B nodes in the 64-B
+	 * class) and 16 B per edge live, plus one cached node per lane; the
+	 * heap adds a superblock per context for each of the two classes
+	 * (nodes, links) */
+	uint64_t need = 144ull * n;
 	for (unsigned t = 0; t < tb_opt.wi_entries && us; t++) {
 		uint64_t threads = tb_threads(&tb_opt.wi[t]);
 		if (threads == 0 || threads > (uint64_t)MAX_WG * 256u) {
@@ -155,7 +156,7 @@ static int run_form(unsigned f, const uint32_t *edges, uint64_t n)
 			continue;
 		}
 		uint32_t nwg = (uint32_t)((threads + 255) / 256);
-		uint64_t bytes = need + 32ull * 256u * nwg;
+		uint64_t bytes = need + 64ull * 256u * nwg;
 		uint64_t nsb = sblocks ? sblocks : 1024u + 2u * (bytes >> 14) + 2u * nctx;
 		if (tb_opt.list) {
 			printf("Threads: %-5" PRIu64 " (%u work-groups of 256)\n", threads, nwg);

@@ -111,3 +111,22 @@ def test_reference_layout_bounded_waits(sim):
     assert out.returncode == 0, out.stdout + out.stderr
     for case in ("deadline", "claim", "exhaust"):
         assert "%s ok" % case in out.stdout, out.stdout
+
+
+@pytest.mark.parametrize("args", [(1, 0), (4, 4), (8, 10), (4, 6, 5), (4, 6, 300)])
+def test_linked_queue_in_simulation(sim, args):
+    """The linked pointer queue (kma_lqueue.h) on CPU threads, with the host
+    AddressSanitizer: every thread enqueues its item, then `rounds` times
+    dequeues one and enqueues it again; the reference's validator
+    (tb_clQueue.c:126-157) then walks one chain from head to the node tail
+    names, holding every item once and the stub once.  With a third argument
+    every call names one of that many queues (hashed per thread and round):
+    a wavefront's calls then split into the lanes served together and lanes
+    served one by one, as a graph's per-node queues see them."""
+    out = subprocess.run([os.path.join(HERE, "_build", "kma_sim_lqueue")] + [str(a) for a in args],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    f = out.stdout.split()
+    v = {k: int(x) for k, x in zip(f[0::2], f[1::2])}
+    assert v["items"] == 256 * args[0] and v["stubs"] == 1 and v["tail_ok"] == 1, out.stdout
+    assert v["cycle"] == 0 and v["outside"] == 0 and v["repeats"] == 0 and v["lost"] == 0, out.stdout

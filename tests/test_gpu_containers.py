@@ -129,6 +129,50 @@ def test_queue_capacity_checked():
         assert len(q.items()) == 0
 
 
+# ------- unbounded linked queue (clQueue.cl:40-162, tb_clQueue.c:126-157) -------
+def _valid(rep, n):
+    """The reference's validator: the chain from head ends, its last node is
+    the one tail names, and it holds each item once (plus the stub)."""
+    assert rep["lost"] == 0 and rep["outside"] == 0 and rep["repeats"] == 0, rep
+    assert rep["cycle"] == 0 and rep["tail_ok"] == 1, rep
+    assert rep["items"] == n and rep["stubs"] == 1, rep
+
+
+def test_lqueue_enqueue_test_kernel():
+    """clQueue_test_enqueue at 100,000 work-items: every item linked once,
+    no capacity of any kind to size."""
+    print("linked queue enqueue", flush=True)
+    with kma_amd.LinkedQueue() as q:
+        rep, _ = q.run(0, 100000)
+        _valid(rep, 100000)
+
+
+@pytest.mark.parametrize("n", [512, 16384])
+def test_lqueue_shuffle_test_kernel(n):
+    """clQueue_test_dequeue (clQueue.cl:140-162): every work-item enqueues its
+    item, then 10 x (dequeue, enqueue again): nothing lost, nothing
+    duplicated (512 work-items: the reference's GLOBAL_SIZE,
+    tb_clQueue.c:31-34)."""
+    print("linked queue shuffle %d" % n, flush=True)
+    with kma_amd.LinkedQueue() as q:
+        rep, _ = q.run(1, n, rounds=10)
+        _valid(rep, n)
+
+
+def test_lqueue_fifo_serial():
+    """One work-item: items come out in the order they went in, and the
+    queue is empty (only its stub left) afterwards; twice, so the stub goes
+    round again."""
+    print("linked queue fifo", flush=True)
+    with kma_amd.LinkedQueue() as q:
+        for _ in range(2):
+            rep, order = q.run(2, 5000)
+            assert np.array_equal(order, np.arange(5000))
+            _valid(rep, 0)
+        rep, order = q.run(2, 0)
+        _valid(rep, 0)
+
+
 # ---------------- array list (clArrayList.cl:32-216) ----------------
 def _pages_and_counts(h, lst):
     pages = lst.pages()
