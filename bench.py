@@ -174,8 +174,10 @@ def bench_stream(reps):
         "null_limit": limit,
         "tuned": {"sb_log2": tune["config"]["sb_log2"], "classes": tune["config"]["classes"],
                   "sblocks": tune["config"]["sblocks"], "npages": tune["config"]["npages"],
-                  "agg": tune["agg"], "nwg": tune["nwg"], "per_thread": tune["per_thread"]},
-        "trials": [{k: t[k] for k in ("sb_log2", "table", "agg", "nwg", "per_thread", "ops_per_s", "nulls", "violations")}
+                  "agg": tune["agg"], "nwg": tune["nwg"], "per_thread": tune["per_thread"],
+                  "no_partial": tune["config"]["no_partial"]},
+        "trials": [{k: t[k] for k in ("sb_log2", "table", "agg", "nwg", "per_thread", "no_partial", "ops_per_s", "nulls",
+                                      "violations")}
                    for t in tune["trials"]],
     }
 

@@ -38,6 +38,8 @@ typedef struct kma_config {
 	uint32_t class_size[KMA_MAX_CLASSES]; /* ascending block sizes, multiples of 8 */
 	uint32_t page_log2;   /* large-request page size log2 (default = sb_log2) */
 	uint64_t npages;      /* pages in the large-request region (0 = none) */
+	uint32_t no_partial;  /* 1: never reattach partly free superblocks (no partial index, no pool-pressure
+				 routing): a detached superblock is reused only once empty, as in the reference */
 } kma_config;
 
 typedef struct kma_heap kma_heap; /* opaque host handle */

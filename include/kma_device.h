@@ -100,7 +100,7 @@ typedef struct kma_dheap {
 	uint32_t pad1;
 	uint64_t off_pidx;     /* partial index: [8 XCDs][nclasses] blocks of pidx_words u64 */
 	uint32_t pidx_words;   /* u64 per (XCD, class) block: levels 0, 1, 2 (see pidx) */
-	uint32_t pad2;
+	uint32_t no_partial;   /* 1: no listing, no taking listed superblocks, no pressure routing */
 } kma_dheap;
 
 /* Per-heap counters (u64 at off_stats). */
@@ -770,6 +770,8 @@ __device__ __forceinline__ void pidx_put(const kma_dheap *H, uint32_t x, uint32_
 /* Take any listed superblock of (x, c) out of the index; false if none. */
 __device__ inline bool pidx_take(const kma_dheap *H, uint32_t x, uint32_t c, uint32_t salt, uint32_t *out)
 {
+	if (H->no_partial)
+		return false;
 	uint64_t *P = pidx(H, x, c);
 	const uint32_t w1 = pidx_w1(H), w2 = (w1 + 63u) >> 6;
 	const uint32_t L1 = pidx_level(H, 1u), L2 = pidx_level(H, 2u);
@@ -823,7 +825,7 @@ __device__ inline bool pidx_remove(const kma_dheap *H, uint32_t x, uint32_t c, u
  * taking listed superblocks). */
 __device__ __forceinline__ bool under_pressure(const kma_dheap *H, uint32_t x)
 {
-	return ld(pressure(H, x)) != 0;
+	return !H->no_partial && ld(pressure(H, x)) != 0;
 }
 
 /* Called by whoever produced state `s` == {count 0, not owned, not listed,
@@ -873,6 +875,8 @@ __device__ inline void put_listed(const kma_dheap *H, uint32_t sb, uint64_t s, u
  * its bit in the index of (home, class) (put_listed). */
 __device__ inline void try_list(const kma_dheap *H, uint32_t sb, uint64_t s, uint32_t lim, uint32_t x, uint32_t ctx)
 {
+	if (H->no_partial)
+		return;
 	const uint32_t c1 = st_cls1(s);
 	for (;;) {
 		if ((s & (ST_OWNED | ST_LISTED)) || st_cls1(s) != c1 || st_count(s) == 0 || st_count(s) > lim)
@@ -1161,7 +1165,14 @@ __device__ __forceinline__ uint32_t nth_bit(uint32_t w, uint32_t k)
  * their rank) spread over different bits instead of all trying the lowest.
  * Words after the first are read with plain atomic loads; after a wrap
  * they are re-read with RMWs, so a stale cached word cannot hide free bits
- * for long.  NULL (counted) when the watchdog expires. */
+ * for long.  NULL (counted) when the watchdog expires.  A caller that holds
+ * a reservation always has a free bit to find: a block's bit is set only
+ * under a reservation and cleared before its count is released, so the
+ * state's count never falls below the set bits and the free bits are at
+ * least the unserved reservations; every lost race is another reservation
+ * served.  The watchdog therefore expires only on a heap whose bitmap was
+ * written outside the allocator, and the slot stays counted (the heap
+ * walker reports the superblock's count/bitmap mismatch). */
 __device__ inline char *claim_scan(const kma_dheap *H, uint32_t sb, uint32_t c, uint32_t wi, uint32_t w,
 				   uint32_t skip)
 {

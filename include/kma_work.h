@@ -239,6 +239,7 @@ int kma_stream_heap_config(const kma_stream *s, uint64_t heap_bytes, uint32_t sb
 typedef struct kma_tune_trial {
 	uint32_t sb_log2, table, nwg, per_thread;
 	uint32_t agg;         /* aggregation width: a KMA_VAR_AGG_* value */
+	uint32_t no_partial;  /* the heap's kma_config.no_partial */
 	double ops_per_s;     /* requests per second */
 	double ms;            /* per replay */
 	uint64_t nulls;
@@ -262,8 +263,10 @@ typedef struct kma_tune_result {
  * work-group staging with 8-, 16- or 32-lane claim groups), work-groups
  * (512, 1024) and requests per thread per chunk (1, 2): 96 trials of `reps`
  * timed replays each, and picks the highest throughput among trials with no
- * violation and at most `null_limit` NULL returns (no limit when 0).  Every
- * trial heap is `heap_bytes`. */
+ * violation and at most `null_limit` NULL returns (no limit when 0); then
+ * one more trial of the pick on a heap with no_partial set, kept if it is
+ * faster under the same conditions (97 trials).  Every trial heap is
+ * `heap_bytes`. */
 int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, uint64_t null_limit,
 			uint32_t reps, kma_tune_result *out);
 

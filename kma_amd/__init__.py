@@ -58,6 +58,7 @@ class Config(ctypes.Structure):
         ("class_size", ctypes.c_uint32 * MAX_CLASSES),
         ("page_log2", ctypes.c_uint32),
         ("npages", ctypes.c_uint64),
+        ("no_partial", ctypes.c_uint32),
     ]
 
 
@@ -156,7 +157,8 @@ class StreamStats(ctypes.Structure):
 class TuneTrial(ctypes.Structure):
     _fields_ = [
         ("sb_log2", ctypes.c_uint32), ("table", ctypes.c_uint32), ("nwg", ctypes.c_uint32),
-        ("per_thread", ctypes.c_uint32), ("agg", ctypes.c_uint32), ("ops_per_s", ctypes.c_double),
+        ("per_thread", ctypes.c_uint32), ("agg", ctypes.c_uint32), ("no_partial", ctypes.c_uint32),
+        ("ops_per_s", ctypes.c_double),
         ("ms", ctypes.c_double),
         ("nulls", ctypes.c_uint64), ("violations", ctypes.c_uint64),
     ]
@@ -174,7 +176,7 @@ class TuneResult(ctypes.Structure):
 def config_dict(c):
     return {"layout": c.layout, "sb_log2": c.sb_log2, "sblocks": c.sblocks, "nctx": c.nctx,
             "npool": c.npool, "classes": list(c.class_size[:c.nclasses]), "page_log2": c.page_log2,
-            "npages": c.npages}
+            "npages": c.npages, "no_partial": c.no_partial}
 
 
 class ServeSpec(ctypes.Structure):
@@ -361,7 +363,7 @@ class Heap:
     """A device heap (kma_create / kma_create_ex)."""
 
     def __init__(self, layout=LAYOUT_REFERENCE, sb_log2=12, sblocks=4096, nctx=0,
-                 npool=0, classes=None, page_log2=0, npages=0):
+                 npool=0, classes=None, page_log2=0, npages=0, no_partial=0):
         c = Config()
         c.layout = layout
         c.sb_log2 = sb_log2
@@ -374,6 +376,7 @@ class Heap:
                 c.class_size[i] = s
         c.page_log2 = page_log2
         c.npages = npages
+        c.no_partial = no_partial
         self._h = lib().kma_create_ex(ctypes.byref(c))
         if not self._h:
             raise KmaError("kma_create_ex: " + last_error())

@@ -102,7 +102,7 @@ typedef struct kma_serve_args {
 	uint64_t duration_ticks;
 	uint64_t cap_blocks;      /* projected admission: block limit (< 2^31) */
 	uint32_t keep_live;       /* skip the drain */
-	uint32_t pad2;
+	uint32_t no_partial;
 	int64_t *slot_job;        /* -1: free */
 	uint32_t *slot_steps;
 	uint64_t *table;          /* [max_live][max_len] block addresses */

@@ -47,6 +47,10 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 		snprintf(err, errlen, "native layout: empty heap");
 		return -1;
 	}
+	if (cfg->no_partial > 1) {
+		snprintf(err, errlen, "native layout: no_partial %u is not 0 or 1", cfg->no_partial);
+		return -1;
+	}
 	if (cfg->sblocks > 0xfffffff0ull) {
 		snprintf(err, errlen, "native layout: too many superblocks");
 		return -1;
@@ -84,6 +88,7 @@ int kma_native_layout(const kma_config *in, kma_config *cfg, nat_hdr *nh, char *
 	nh->magic = NAT_MAGIC;
 	nh->nsb = cfg->sblocks;
 	nh->npages = cfg->npages;
+	nh->no_partial = cfg->no_partial;
 	nh->sb_log2 = cfg->sb_log2;
 	nh->page_log2 = cfg->page_log2;
 	nh->nclasses = cfg->nclasses;

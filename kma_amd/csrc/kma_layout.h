@@ -47,7 +47,7 @@ typedef struct nat_hdr {
 	uint32_t pad1;
 	uint64_t off_pidx;
 	uint32_t pidx_words;
-	uint32_t pad2;
+	uint32_t no_partial;
 } nat_hdr;
 
 #define NAT_MAGIC 0x4e414d4b31414d4bull

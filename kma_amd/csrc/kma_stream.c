@@ -543,9 +543,42 @@ int kma_stream_autotune(kma_stream *s, uint64_t heap_bytes, uint32_t page_log2, 
 		set_err("every trial had violations");
 		return -1;
 	}
+	/* last trial: the chosen configuration without reuse of partly free
+	 * superblocks (on a trace that keeps the heap oversubscribed the reuse
+	 * can cost more refills than the capacity it recovers) */
+	{
+		const kma_tune_trial b = out->trials[best];
+		kma_config cfg;
+		if (kma_stream_heap_config(s, heap_bytes, b.sb_log2, b.table, page_log2, &cfg) != 0)
+			return -1;
+		cfg.no_partial = 1;
+		kma_heap *h = kma_create_ex(&cfg);
+		if (!h)
+			return -1;
+		kma_stream_stats st;
+		if (out->ntrials >= KMA_TUNE_MAX ||
+		    kma_stream_run(h, s, KMA_FORM_BATCHED, b.nwg, b.per_thread, 1, reps, b.agg, &st) != 0) {
+			kma_destroy(h);
+			return -1;
+		}
+		kma_destroy(h);
+		kma_tune_trial *t = &out->trials[out->ntrials];
+		*t = b;
+		t->no_partial = 1;
+		t->ms = st.gpu_ms;
+		t->ops_per_s = st.gpu_ms > 0 ? (double)st.ops / (st.gpu_ms * 1e-3) : 0;
+		t->nulls = st.nulls;
+		t->violations = st.bad + st.wait_timeouts + st.double_frees + st.bad_frees + st.waits[0] +
+				st.waits[1] + st.waits[2] + st.waits[3];
+		const int i = (int)out->ntrials++;
+		if (!t->violations && (null_limit == 0 || t->nulls <= null_limit) &&
+		    t->ops_per_s > out->trials[best].ops_per_s)
+			best = i;
+	}
 	const kma_tune_trial *t = &out->trials[best];
 	if (kma_stream_heap_config(s, heap_bytes, t->sb_log2, t->table, page_log2, &out->cfg) != 0)
 		return -1;
+	out->cfg.no_partial = t->no_partial;
 	out->nwg = t->nwg;
 	out->per_thread = t->per_thread;
 	out->table = t->table;

@@ -12,7 +12,7 @@ HDR_DTYPE = np.dtype([
     ("class_size", "<u4", 32), ("class_cap", "<u4", 32), ("class_magic", "<u8", 32),
     ("class_for_log2", "u1", 64),
     ("off_psum", "<u8"), ("off_spare", "<u8"), ("spare_slots", "<u4"), ("pad1", "<u4"),
-    ("off_pidx", "<u8"), ("pidx_words", "<u4"), ("pad2", "<u4"),
+    ("off_pidx", "<u8"), ("pidx_words", "<u4"), ("no_partial", "<u4"),
 ])
 
 MAGIC = 0x4E414D4B31414D4B

@@ -15,9 +15,10 @@ def _clean(h):
     return r
 
 
-def test_fresh_and_after_workloads():
-    print("heap check: fresh + workloads", flush=True)
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 4096, npages=1000) as h:
+@pytest.mark.parametrize("no_partial", [0, 1])
+def test_fresh_and_after_workloads(no_partial):
+    print("heap check: fresh + workloads, no_partial=%d" % no_partial, flush=True)
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 4096, npages=1000, no_partial=no_partial) as h:
         r = _clean(h)
         assert r["sb_free"] == r["sb_total"] == 4096 and r["pages_total"] == 1000
         for form in (kma_amd.FORM_PLAIN, kma_amd.FORM_FUSED, kma_amd.FORM_BATCHED):

@@ -192,8 +192,9 @@ def test_reference_kernels_full_occupancy_native(mode):
         assert natimg.NatHeap(h).canonical_problems() == []
 
 
+@pytest.mark.parametrize("no_partial", [1, 0])
 @pytest.mark.parametrize("form", [kma_amd.FORM_PLAIN, kma_amd.FORM_BATCHED])
-def test_cross_xcd_supply_by_migration(form):
+def test_cross_xcd_supply_by_migration(form, no_partial):
     """A heap of 16 superblocks (two per XCD, four 4 KiB blocks each) under
     eight work-groups of 4 KiB requests: superblocks emptied on one XCD stay
     in its dirty shards, so the others get them only through in-kernel
@@ -201,16 +202,23 @@ def test_cross_xcd_supply_by_migration(form):
     no watchdog fires, and the heap is canonical after trim.  The per-lane
     form requests migrations (its superblocks idle in dirty shards while the
     owner runs); the collective form takes whole superblocks and mostly
-    re-pops its own dirty ones, so migrations there are timing-dependent."""
-    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16, nctx=8, npool=16) as h:
-        _say("native cross-XCD supply form=%d" % form)
+    re-pops its own dirty ones, so migrations there are timing-dependent.
+    With partial reuse on (no_partial 0) an XCD under pool pressure also
+    refills from its own partly free superblocks, which can serve every
+    refill of a run: then some superblock is migrated or reattached."""
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16, nctx=8, npool=16, no_partial=no_partial) as h:
+        _say("native cross-XCD supply form=%d no_partial=%d" % (form, no_partial))
         st = h.run(kma_amd.WL_FIXED, 8 * 256, 30, form=form, block=256, fixed=4096)
         assert st["ops"] >= 30 * 8 * 256 and st["bad"] == 0, st
         assert st["double_frees"] == 0 and st["bad_frees"] == 0, st
         assert st["waits"] == [0, 0, 0, 0], st
         assert 0 < st["nulls"] < st["mallocs"], st
-        if form == kma_amd.FORM_PLAIN:
+        if form == kma_amd.FORM_PLAIN and no_partial:
             assert st["migrated"] > 0, st
+        elif form == kma_amd.FORM_PLAIN:
+            assert st["migrated"] + st["reattached"] > 0, st
+        if no_partial:
+            assert st["listed"] == 0 and st["reattached"] == 0, st
         h.trim()
         assert natimg.NatHeap(h).canonical_problems() == []
 

@@ -18,10 +18,10 @@ pytestmark = pytest.mark.gpu
 CLASSES = [16 << i for i in range(11)]  # 16 B .. 16 KiB
 
 
-def stream_heap(sblocks=2048, npages=4096, sb_log2=14):
+def stream_heap(sblocks=2048, npages=4096, sb_log2=14, no_partial=0):
     return kma_amd.Heap(layout=kma_amd.LAYOUT_NATIVE, sb_log2=sb_log2, sblocks=sblocks,
                         classes=[c for c in CLASSES if c <= (1 << sb_log2)], page_log2=14,
-                        npages=npages)
+                        npages=npages, no_partial=no_partial)
 
 
 def extents(nh, offs, sizes):
@@ -125,12 +125,14 @@ def test_stream_invariants(form, nwg, per_thread):
         assert nh.canonical_problems() == []
 
 
+@pytest.mark.parametrize("no_partial", [0, 1])
 @pytest.mark.parametrize("form", [kma_amd.FORM_PLAIN, kma_amd.FORM_BATCHED])
-def test_stream_oversubscribed(form):
+def test_stream_oversubscribed(form, no_partial):
     """A heap far smaller than the live set: NULLs, and still every
-    invariant and the empty-heap image afterwards."""
-    print("stream oversubscribed form=%d" % form, flush=True)
-    with stream_heap(sblocks=48, npages=96) as h, kma_amd.Stream(40, 4096, seed=5) as s:
+    invariant and the empty-heap image afterwards; with and without reuse
+    of partly free superblocks."""
+    print("stream oversubscribed form=%d no_partial=%d" % (form, no_partial), flush=True)
+    with stream_heap(sblocks=48, npages=96, no_partial=no_partial) as h, kma_amd.Stream(40, 4096, seed=5) as s:
         st = s.run(h, form=form, nwg=128, per_thread=2, variant=kma_amd.VAR_LOG)
         check_run(st, s.info)
         assert st["nulls"] > 0 and st["large_nulls"] > 0, st
@@ -185,12 +187,20 @@ def test_stream_autotune_small():
     print("stream autotune", flush=True)
     with kma_amd.Stream(48, 4096, seed=9) as s:
         r = s.autotune(256 << 20, 14, null_limit=0, reps=1)
-        assert len(r["trials"]) == 96
+        assert len(r["trials"]) == 97
         assert all(t["violations"] == 0 for t in r["trials"]), r["trials"]
         best = max(r["trials"], key=lambda t: t["ops_per_s"])
         assert r["ops_per_s"] == best["ops_per_s"]
         assert r["config"]["sb_log2"] == best["sb_log2"] and r["nwg"] == best["nwg"] and r["agg"] == best["agg"]
+        assert r["config"]["no_partial"] == best["no_partial"]
         assert {t["agg"] for t in r["trials"]} == {0x00, 0x10, 0x20, 0x30}
+        # the last trial repeats the best of the first 96 on a heap without
+        # partial reuse
+        first = max(r["trials"][:96], key=lambda t: t["ops_per_s"])
+        last = r["trials"][96]
+        assert last["no_partial"] == 1 and all(t["no_partial"] == 0 for t in r["trials"][:96])
+        assert {k: last[k] for k in ("sb_log2", "table", "agg", "nwg", "per_thread")} == \
+            {k: first[k] for k in ("sb_log2", "table", "agg", "nwg", "per_thread")}
         with kma_amd.Heap.from_config(r["cfg"]) as h:
             st = s.run(h, form=kma_amd.FORM_BATCHED, nwg=r["nwg"], per_thread=r["per_thread"], variant=r["agg"])
             check_run(st, s.info)
