@@ -14,7 +14,9 @@
  *   variant 0: per-lane kma_malloc / kma_free; 1: kma_wg_malloc / kma_wg_free
  *   with work-group staging (16-lane claim groups)
  *   KMA_SIM_DUMP=path writes the metadata image before the final frees to
- *   path.run (live blocks and listed superblocks present)
+ *   path.run (live blocks and listed superblocks present), the image of the
+ *   fresh heap to path.init and the image after the final frees and the
+ *   trim to path.final
  * prints: mallocs nulls bad peak_live capacity listed reattached canonical_problems
  */
 #include <stdio.h>
@@ -100,6 +102,16 @@ void trim(kma_dheap *H)
 	clean_all(H);
 }
 
+bool dump_image(const char *dump, const char *suffix, const char *buf, uint64_t n)
+{
+	std::string name = std::string(dump) + suffix;
+	FILE *f = fopen(name.c_str(), "wb");
+	bool ok = f && fwrite(buf, 1, n, f) == n;
+	if (f)
+		ok = fclose(f) == 0 && ok;
+	return ok;
+}
+
 } // namespace
 
 int main(int argc, char **argv)
@@ -142,6 +154,10 @@ int main(int argc, char **argv)
 		const uint64_t n = kma_nat::init_items(H->nsb, H->npool, H->npages);
 		for (uint64_t i = 0; i < n; i++)
 			kma_nat::init_item(H, i);
+	}
+	if (dump && !dump_image(dump, ".init", buf, nh.off_data)) {
+		fprintf(stderr, "cannot write the image\n");
+		return 3;
 	}
 	const uint32_t size = H->class_size[0];
 	const uint64_t capacity = H->nsb * H->class_cap[0];
@@ -239,6 +255,10 @@ int main(int argc, char **argv)
 	const uint64_t listed = st[KMA_STAT_LISTED], reattached = st[KMA_STAT_REATTACHED];
 	trim(H);
 	const int probs = canonical_problems(H);
+	if (dump && !dump_image(dump, ".final", buf, nh.off_data)) {
+		fprintf(stderr, "cannot write the image\n");
+		return 3;
+	}
 	printf("mallocs %llu nulls %llu bad %llu peak_live %llu capacity %llu listed %llu reattached %llu "
 	       "double_frees %llu waits %llu canonical_problems %d\n",
 	       (unsigned long long)mallocs.load(), (unsigned long long)nulls.load(), (unsigned long long)bad.load(),

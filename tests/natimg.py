@@ -182,3 +182,69 @@ def check_blocks(nh, offs, sizes):
         if a[1] > b[0]:
             probs.append("overlap %#x..%#x / %#x" % (a[0], a[1], b[0]))
     return probs
+
+
+# ---- byte comparison with the initial image ----
+#
+# After a trace that frees every block, followed by kma_trim, the metadata
+# image must equal the image of a freshly created heap in every bit except
+# these (the exclusion mask), each of which legitimately carries history:
+#   * the statistics counters (256 B at off_stats);
+#   * per XCD (128-B lines after them): the shard hint mask, the clean
+#     request word, the exhaustion window and the pool-pressure word (words
+#     0..3; the rest of each line must match);
+#   * the generation of every superblock state word (bits 42..63: the ABA
+#     guard advances on every release);
+#   * the pool: each shard's {tag, top} head and count (words 0 and 1 of its
+#     128-B line; the rest must match) and the pool links (which shard holds
+#     which superblock, in which order, follows the release order);
+#   * the page-run lengths (a freed run's length stays behind; it is read
+#     only where its run's first bit is set).
+# The excluded pool structure is checked separately: every superblock in
+# exactly one shard, every non-empty shard with its hint bit
+# (NatHeap.canonical_problems).
+
+def restore_mask(h):
+    """Per-byte bit mask over the metadata image [0, off_data) of header `h`
+    (HDR_DTYPE): set bits may differ from the initial image."""
+    n = int(h["off_data"])
+    m = np.zeros(n, np.uint8)
+    so = int(h["off_stats"])
+    m[so: so + 256] = 0xFF
+    for x in range(8):
+        line = so + 256 + 128 * x
+        m[line: line + 32] = 0xFF
+    nsb = int(h["nsb"])
+    st = m[int(h["off_state"]): int(h["off_state"]) + 8 * nsb].reshape(nsb, 8)
+    st[:, 5] |= 0xFC  # bits 42..47
+    st[:, 6:8] = 0xFF  # bits 48..63
+    m[int(h["off_next"]): int(h["off_next"]) + 4 * nsb] = 0xFF
+    npool = int(h["npool"])
+    pool = m[int(h["off_pool"]): int(h["off_pool"]) + 128 * npool].reshape(npool, 128)
+    pool[:, 0:16] = 0xFF
+    m[int(h["off_plen"]): int(h["off_plen"]) + int(h["npages"])] = 0xFF
+    return m
+
+
+def _region(h, off):
+    """Name of the table that holds byte `off` of the metadata image."""
+    keys = ["off_stats", "off_state", "off_next", "off_bitmap", "off_heads", "off_pool", "off_pbitmap",
+            "off_plen", "off_psum", "off_spare", "off_pidx", "off_data"]
+    starts = sorted((int(h[k]), k[4:]) for k in keys)
+    name = "header"
+    for s, k in starts:
+        if off >= s:
+            name = k
+    return name
+
+
+def restored_diff(img0, img1):
+    """Bytes of metadata image img1 that differ from the initial image img0
+    outside the exclusion mask: a list of (offset, table, initial, now), at
+    most 20.  Both are uint8 arrays of at least off_data bytes."""
+    h = np.frombuffer(img0[: HDR_DTYPE.itemsize].tobytes(), dtype=HDR_DTYPE)[0]
+    n = int(h["off_data"])
+    a, b = np.asarray(img0[:n], np.uint8), np.asarray(img1[:n], np.uint8)
+    diff = (a ^ b) & ~restore_mask(h)
+    idx = np.nonzero(diff)[0]
+    return [(int(i), _region(h, int(i)), int(a[i]), int(b[i])) for i in idx[:20]]

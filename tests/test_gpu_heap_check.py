@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import kma_amd
+from tests import natimg
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +34,24 @@ def test_fresh_and_after_workloads(no_partial):
         h.trim()
         r = _clean(h)
         assert r["sb_free"] == 4096 and r["sb_in_spares"] == 0
+
+
+@pytest.mark.parametrize("no_partial", [0, 1])
+def test_metadata_restored_byte_for_byte(no_partial):
+    """All three forms of the full-occupancy workload, then kma_trim: the
+    metadata image equals the fresh heap's in every bit outside the
+    documented exclusion mask (natimg.restore_mask)."""
+    print("heap check: byte comparison, no_partial=%d" % no_partial, flush=True)
+    with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 4096, npages=1000, no_partial=no_partial) as h:
+        nh = natimg.NatHeap(h)
+        init = np.asarray(h.snapshot(0, nh.off_data), dtype=np.uint8).copy()
+        for form in (kma_amd.FORM_PLAIN, kma_amd.FORM_FUSED, kma_amd.FORM_BATCHED):
+            st = h.run(kma_amd.WL_LOGUNI, 65536, 4, form=form, block=256, seed=7)
+            assert st["nulls"] == 0 and st["bad"] == 0, st
+        h.trim()
+        final = np.asarray(h.snapshot(0, nh.off_data), dtype=np.uint8)
+        assert natimg.restored_diff(init, final) == []
+        assert nh.canonical_problems() == []
 
 
 @pytest.mark.parametrize("mode", [kma_amd.GRAPH_PLAIN, kma_amd.GRAPH_CACHE, kma_amd.GRAPH_COLLECTIVE])

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import kma_amd
+from tests import natimg
 from tests.natimg import NatHeap
 
 pytestmark = pytest.mark.gpu
@@ -163,6 +164,23 @@ def test_stream_large_page_region():
         assert not lifetime_overlaps(req, res, ticks, nh)
         h.trim()
         assert nh.canonical_problems() == []
+
+
+def test_stream_restores_metadata_byte_for_byte():
+    """Replays of a draining stream with page runs and NULLs, then kma_trim:
+    the metadata image equals the fresh heap's in every bit outside the
+    documented exclusion mask (natimg.restore_mask), page bitmap and
+    page-window summaries included."""
+    print("stream byte comparison", flush=True)
+    with stream_heap(sblocks=96, npages=256) as h, kma_amd.Stream(40, 4096, seed=5) as s:
+        nh = NatHeap(h)
+        init = np.asarray(h.snapshot(0, nh.off_data), dtype=np.uint8).copy()
+        st = s.run(h, form=kma_amd.FORM_BATCHED, nwg=128, per_thread=2, warmup=1, reps=2)
+        check_run(st, s.info)
+        assert st["large"] > 0 and st["nulls"] > 0, st
+        h.trim()
+        final = np.asarray(h.snapshot(0, nh.off_data), dtype=np.uint8)
+        assert natimg.restored_diff(init, final) == []
 
 
 def test_stream_replays_return_heap_to_empty():
