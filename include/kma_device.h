@@ -1040,12 +1040,19 @@ struct resv {
  * keep that lane masked off until the watchdog, and with every class of a
  * wavefront served at once, a wavefront holding one head while a lane of
  * it waits on another can close a cycle with a second wavefront. */
-__device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c, uint32_t n)
+/* `hint`, if it names a superblock, is taken as the head's value without
+ * reading the head (a value the caller saw earlier: it is checked like a
+ * stale read, by the generation and class the fadd returns); *used gets the
+ * head value the returned slots came from (0 when none). */
+__device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t ctx, uint32_t c, uint32_t n,
+					uint64_t hint = 0, uint64_t *used = nullptr)
 {
 	uint64_t *hp = head(H, ctx, c);
 	uint32_t cap = H->class_cap[c];
 	waiter wt;
-	uint64_t h = ld(hp);
+	uint64_t h = hint > HEAD_BUSY && (uint32_t)hint - 2u < H->nsb ? hint : ld(hp);
+	if (used)
+		*used = 0;
 	for (;;) {
 		if (wt.expired()) {
 			bump_stat(H, KMA_STAT_WAIT_RESERVE);
@@ -1087,6 +1094,8 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 			st(hp, h);
 			if (got == 0)
 				continue;
+			if (used)
+				*used = h;
 			return resv{sb, cnt, got, 1};
 		}
 		if (h == HEAD_BUSY) {
@@ -1116,6 +1125,8 @@ __device__ __forceinline__ resv reserve(const kma_dheap *H, uint32_t x, uint32_t
 				continue;
 			}
 		}
+		if (used)
+			*used = h;
 		return resv{sb, cnt, got, 1};
 	}
 }

@@ -42,7 +42,23 @@ struct kma_wg_lds {
 	unsigned long long hkey[2 * BLOCK];
 	uint32_t hmask[2 * BLOCK];
 	uint32_t hn[2 * BLOCK];
+	/* per class: the context head value this work-group's last reservation
+	 * of the class came from (0: none); the next reservation tries that
+	 * superblock before reading the head.  kma_wg_lds_init clears it;
+	 * left uninitialised it only costs first attempts (any value is
+	 * checked like a stale head read). */
+	uint64_t scache[32];
 };
+
+/* Clear the work-group's superblock cache; every thread calls it once,
+ * before the work-group's first collective call. */
+template <int BLOCK>
+__device__ __forceinline__ void kma_wg_lds_init(kma_wg_lds<BLOCK> &L)
+{
+	if (threadIdx.x < 32u)
+		L.scache[threadIdx.x] = 0;
+	__syncthreads();
+}
 
 namespace kma_nat {
 
@@ -364,8 +380,11 @@ __device__ __forceinline__ void *kma_wg_malloc(kma_dheap *H, size_t size, kma_wg
 				break;
 			}
 			resv r = {0, 0, 0, 0};
-			if (pend && gl == 0)
-				r = reserve(H, x, ctx, k, n - r0);
+			if (pend && gl == 0) {
+				uint64_t used;
+				r = reserve(H, x, ctx, k, n - r0, L.scache[k], &used);
+				L.scache[k] = used;
+			}
 			const uint32_t ok = group_bcast<G>(r.ok), sb = group_bcast<G>(r.sb);
 			const uint32_t rb = group_bcast<G>(r.base), got = group_bcast<G>(r.got);
 			const bool want = pend && ok;

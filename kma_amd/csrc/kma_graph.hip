@@ -230,6 +230,7 @@ __global__ void __launch_bounds__(GBLOCK) k_graph_wg(kma_dheap *H, kma_tree *t, 
 						     unsigned long long *stats)
 {
 	__shared__ kma_wg_lds<GBLOCK> L;
+	kma_wg_lds_init(L);
 	gcount c;
 	gnode *cache = nullptr;
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

@@ -342,6 +342,7 @@ template <bool GENERIC, uint32_t AGG>
 __global__ void __launch_bounds__(KMA_BATCH_BLOCK) k_batched(kma_dheap *H, kma_wl_args a)
 {
 	__shared__ kma_wg_lds<KMA_BATCH_BLOCK> L;
+	kma_wg_lds_init(L);
 	wl_batched<GENERIC, AGG>(H, L, a);
 }
 

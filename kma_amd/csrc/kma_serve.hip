@@ -502,6 +502,7 @@ __global__ void __launch_bounds__(SV_BLOCK) k_serve(kma_serve_args a)
 	__shared__ kma_wg_lds<SV_BLOCK> L;
 	__shared__ sv_admit_lds P; /* used by work-group 0 only */
 	__shared__ uint32_t s_max;
+	kma_wg_lds_init(L);
 	const uint32_t tid = threadIdx.x, nwg = gridDim.x;
 	const bool wg0 = blockIdx.x == 0;
 	uint64_t *ctl = a.ctl;

@@ -221,6 +221,7 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, cons
 	using namespace kma_nat;
 	__shared__ kma_wg_lds<SBLOCK> L;
 	__shared__ uint64_t s_base;
+	kma_wg_lds_init(L);
 	const uint32_t ep = (uint32_t)ctl[KMA_CTL_EPOCH];
 	const uint32_t tid = threadIdx.x;
 	const bool log = (variant & KMA_VAR_LOG) != 0;
