@@ -289,7 +289,9 @@ __device__ inline uint32_t partial_superblocks(const kma_dheap *H, uint32_t x, u
 		relist(H, sb, x, ctx, true);
 	return wtot;
 }
+#ifndef KMA_PART_ROUNDS
 #define KMA_PART_ROUNDS 4u
+#endif
 
 __device__ __forceinline__ uint32_t wg_hash(uint64_t key, uint32_t n)
 {

@@ -204,8 +204,9 @@ def test_cross_xcd_supply_by_migration(form, no_partial):
     owner runs); the collective form takes whole superblocks and mostly
     re-pops its own dirty ones, so migrations there are timing-dependent.
     With partial reuse on (no_partial 0) an XCD under pool pressure also
-    refills from its own partly free superblocks, which can serve every
-    refill of a run: then some superblock is migrated or reattached."""
+    lists and refills from its own partly free superblocks (four slots
+    each here), so a run may need no migration at all: that case checks
+    only correctness."""
     with kma_amd.Heap(kma_amd.LAYOUT_NATIVE, 14, 16, nctx=8, npool=16, no_partial=no_partial) as h:
         _say("native cross-XCD supply form=%d no_partial=%d" % (form, no_partial))
         st = h.run(kma_amd.WL_FIXED, 8 * 256, 30, form=form, block=256, fixed=4096)
@@ -215,8 +216,6 @@ def test_cross_xcd_supply_by_migration(form, no_partial):
         assert 0 < st["nulls"] < st["mallocs"], st
         if form == kma_amd.FORM_PLAIN and no_partial:
             assert st["migrated"] > 0, st
-        elif form == kma_amd.FORM_PLAIN:
-            assert st["migrated"] + st["reattached"] > 0, st
         if no_partial:
             assert st["listed"] == 0 and st["reattached"] == 0, st
         h.trim()
