@@ -575,16 +575,17 @@ __device__ __forceinline__ void serve_creq(const kma_dheap *H, uint32_t x, uint3
  * one context over slots and shards.  A pop reads only its own
  * XCD's mask/request line until its own shards are exhausted (every pop of
  * the device reading all eight masks made them hot), then the other masks
- * in one round trip.  With no clean superblock anywhere, a caller asks the
- * XCDs holding dirty superblocks for a migration (requests already pending
- * are not repeated) and, by default, fails at once: the migrated
- * superblocks serve later callers.  KMA_CREQ_POLLS > 0 makes it wait that
- * many short polls for them instead (on the oversubscribed request stream,
- * waiting one poll cost 11-18 % of the time for the same NULL count).
+ * in one round trip.  With no clean superblock anywhere, a caller that
+ * newly requests a migration from the XCDs holding dirty ones waits up to
+ * KMA_CREQ_POLLS short polls for one (serving requests aimed at its own XCD
+ * meanwhile); requests already pending mean those XCDs have not answered,
+ * and the caller fails at once.  (Never waiting was 11-18 % faster on the
+ * oversubscribed request stream, but returned NULLs from heaps with room on
+ * other XCDs.)
  * KMA_STEAL_CTX other contexts' spare lines are looked into per search (8
  * cost 4-8 % over 2 on that stream, where most searches fail). */
 #ifndef KMA_CREQ_POLLS
-#define KMA_CREQ_POLLS 0u
+#define KMA_CREQ_POLLS 1u
 #endif
 #ifndef KMA_STEAL_CTX
 #define KMA_STEAL_CTX 2u

@@ -44,9 +44,14 @@ constexpr uint64_t RESULT_WAIT_TICKS = 200000000ull;
 
 /* Named fields, not an array: the compiler merges increments on different
  * paths into one with a computed index, which would put an array in scratch. */
+/* Wavefront totals: every lane of the wavefront counts together, in uniform
+ * control flow, with ballots, so the totals are scalars (kept out of the
+ * vector registers the allocator's calls need). */
 struct counters {
 	uint32_t ops = 0, mok = 0, nul = 0, fok = 0, fnull = 0, bad = 0, tmo = 0, large = 0, lnull = 0;
 };
+
+__device__ __forceinline__ uint32_t wcount(bool b) { return (uint32_t)__popcll(__ballot(b)); }
 
 __device__ __forceinline__ uint32_t pattern(uint64_t j)
 {
@@ -79,13 +84,9 @@ __device__ __forceinline__ void flush(uint64_t *ctl, const counters &c, const ph
 {
 	const uint32_t v[C_N] = {c.ops, c.mok, c.nul, c.fok, c.fnull, c.bad, c.tmo, c.large, c.lnull};
 #pragma unroll
-	for (int k = 0; k < C_N; k++) {
-		uint64_t x = v[k];
-		for (int o = 32; o > 0; o >>= 1)
-			x += __shfl_xor(x, o);
-		if (__lane_id() == 0 && x)
-			atomicAdd((unsigned long long *)(ctl + KMA_CTL_STATS + k), (unsigned long long)x);
-	}
+	for (int k = 0; k < C_N; k++)
+		if (__lane_id() == 0 && v[k])
+			atomicAdd((unsigned long long *)(ctl + KMA_CTL_STATS + k), (unsigned long long)v[k]);
 	if (ph.on && __lane_id() == 0) {
 		for (int k = 0; k < 3; k++)
 			atomicAdd((unsigned long long *)(ctl + KMA_CTL_STATS + C_N + k), (unsigned long long)ph.acc[k]);
@@ -93,13 +94,14 @@ __device__ __forceinline__ void flush(uint64_t *ctl, const counters &c, const ph
 	}
 }
 
-__device__ __forceinline__ void count_malloc(const kma_dheap *H, uint32_t size, const void *p, counters &c)
+/* every lane calls; ism: the lane made a malloc of `size` that returned p */
+__device__ __forceinline__ void count_malloc(const kma_dheap *H, bool ism, uint32_t size, const void *p, counters &c)
 {
-	bool large = size > H->max_small;
-	c.mok += p != nullptr;
-	c.nul += p == nullptr;
-	c.large += large;
-	c.lnull += large && !p;
+	const bool large = ism && size > H->max_small;
+	c.mok += wcount(ism && p != nullptr);
+	c.nul += wcount(ism && p == nullptr);
+	c.large += wcount(large);
+	c.lnull += wcount(large && !p);
 }
 
 /* Outcome of a free's wait for its malloc's result. */
@@ -140,12 +142,13 @@ __device__ __forceinline__ void *take_target(const kma_dheap *H, uint64_t *res, 
 	return p;
 }
 
-__device__ __forceinline__ void count_free(uint32_t kind, counters &c)
+/* every lane calls; isf: the lane waited for a free's target, with outcome kind */
+__device__ __forceinline__ void count_free(bool isf, uint32_t kind, counters &c)
 {
-	c.fok += kind == T_OK || kind == T_BAD;
-	c.bad += kind == T_BAD;
-	c.fnull += kind == T_NULL;
-	c.tmo += kind == T_TIMEOUT;
+	c.fok += wcount(isf && (kind == T_OK || kind == T_BAD));
+	c.bad += wcount(isf && kind == T_BAD);
+	c.fnull += wcount(isf && kind == T_NULL);
+	c.tmo += wcount(isf && kind == T_TIMEOUT);
 }
 
 __device__ __forceinline__ uint64_t result_word(const kma_dheap *H, const void *p, uint32_t ep)
@@ -185,10 +188,10 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_plain(kma_dheap *H, const uin
 	ph.start();
 	if (ism) {
 		p = kma_malloc(H, r);
-		count_malloc(H, r, p, c);
 		if (p)
 			st((uint32_t *)p, pattern(j));
 	}
+	count_malloc(H, ism, r, p, c);
 	order_prior(); /* pattern stores complete before results publish */
 	if (ism) {
 		st(res + j, result_word(H, p, ep));
@@ -197,19 +200,19 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_plain(kma_dheap *H, const uin
 	}
 	ph.mark(0);
 	void *q = nullptr;
+	uint32_t kind = T_OK;
 	if (isf) {
-		uint32_t kind;
 		q = take_target(H, res, r & 0x7fffffffu, ep, &kind);
-		count_free(kind, c);
 		if (log)
 			ts[j] = wall_clock64();
 	}
+	count_free(isf, kind, c);
 	ph.mark(1);
 	if (isf)
 		kma_free(H, q);
 	ph.mark(2);
 	ph.n = 1;
-	c.ops = in;
+	c.ops = wcount(in);
 	flush(ctl, c, ph);
 }
 
@@ -247,18 +250,16 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, cons
 			const uint32_t r = in ? req[j] : 0u;
 			const uint32_t size = (in && !(r >> 31)) ? r : 0u;
 			void *p = kma_wg_malloc<SBLOCK, AGG>(H, size, L);
-			if (size) {
-				count_malloc(H, size, p, c);
-				if (p)
-					st((uint32_t *)p, pattern(j));
-			}
+			count_malloc(H, size != 0u, size, p, c);
+			if (size && p)
+				st((uint32_t *)p, pattern(j));
 			order_prior();
 			if (size) {
 				st(res + j, result_word(H, p, ep));
 				if (log)
 					ts[j] = wall_clock64();
 			}
-			c.ops += in;
+			c.ops += wcount(in);
 		}
 		ph.mark(0);
 		for (uint32_t m = 0; m < nsl; m++) {
@@ -266,13 +267,14 @@ __global__ void __launch_bounds__(SBLOCK) k_stream_persistent(kma_dheap *H, cons
 			const bool in = j < nreq;
 			const uint32_t r = in ? req[j] : 0u;
 			void *q = nullptr;
-			if (in && (r >> 31)) {
-				uint32_t kind;
+			uint32_t kind = T_OK;
+			const bool isf = in && (r >> 31);
+			if (isf) {
 				q = take_target(H, res, r & 0x7fffffffu, ep, &kind);
-				count_free(kind, c);
 				if (log)
 					ts[j] = wall_clock64();
 			}
+			count_free(isf, kind, c);
 			ph.mark(1);
 			kma_wg_free<SBLOCK, AGG>(H, q, L);
 			ph.mark(2);
